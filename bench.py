@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""SPME reciprocal-space evaluations/s on one MI355X.
+
+One step = one evaluation (sfac + de1 + de2 + potl: spreading, r2c FFT, /B,
+influence multiply with energy/virial, c2r FFT, force + potential gather)
+of the water_98k workload (32,768 rigid 3-site waters = 98,304 atoms, 100 A
+cube, 128^3 grid, B-spline order 4, beta 0.32), with positions and charges
+already resident in HBM.  Influence-table setup (set_A) is outside the
+timed region, as in the reference's usage.
+
+Prints ONE JSON line.  `value` is the FP32 engine's evaluations/s; the FP64
+engine is measured in the same run and reported beside it, with the
+per-evaluation latency distribution (p50/p99) of both.
+
+    python bench.py [--steps K] [--warmup W] [--workload water_98k]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("SPME recip evals/s at 98k atoms, FP32 vs FP64 GPU; p50/p99 batched latency")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from ewaldcorrel_amd import workloads as wl
+    if name == "water_98k":
+        return wl.water_98k()
+    if name == "point_charges_1k":
+        return wl.point_charges_1k()
+    if name == "gaussian_1m":
+        return wl.gaussian_1m()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def kernel_bytes(kernel, n, K, p, pg):
+    """Algorithmic HBM bytes of one launch: the compulsory reads and writes of
+    that kernel.  p = working-precision bytes, pg = grid-accumulation bytes;
+    positions/charges/forces/potentials are doubles at the ABI (8 B)."""
+    K0, K1, K2 = K
+    G = K0 * K1 * K2
+    H = K0 * K1 * (K2 // 2 + 1)
+    table = {
+        "spread": n * 4 * 8 + G * pg,
+        "fixed_to_grid": G * pg + G * p,
+        "zero_grid": 2 * H * p,
+        "fft_z_r2c": G * p + 2 * H * p,
+        "fft_y_fwd": 4 * H * p,
+        "fft_x_fwd": 4 * H * p,
+        "influence": 2 * H * p + 2 * H * p + 8 * H,
+        "fft_x_inv": 4 * H * p,
+        "fft_y_inv": 4 * H * p,
+        "fft_z_c2r": 2 * H * p + G * p,
+        "gather": G * p + n * 4 * 8 + n * 4 * 8,
+    }
+    return table.get(kernel)
+
+
+def eval_bytes(n, K, p, pg):
+    """B_eval: spread + fft_fwd + infl + fft_inv + gather (compulsory traffic
+    of one evaluation, FP64 influence table at 8 B per half-complex point)."""
+    K0, K1, K2 = K
+    G = K0 * K1 * K2
+    H = K0 * K1 * (K2 // 2 + 1)
+    spread = n * 4 * p + G * pg
+    fwd = G * p + 2 * H * p
+    infl = 2 * H * p + 2 * H * p + 8 * H
+    inv = 2 * H * p + G * p
+    gather = G * p + n * 4 * p + n * 4 * p
+    return spread + fwd + infl + inv + gather
+
+
+def time_engine(sy, precision, steps, warmup, profile):
+    import torch
+    from ewaldcorrel_amd import ewald
+    dev = torch.device("cuda:0")
+    s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
+    s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
+    n = sy.n
+    w = torch.tensor(sy.w, dtype=torch.float64, device=dev)
+    x = torch.tensor(sy.x, dtype=torch.float64, device=dev)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+    dx = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    phi = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(s.stream)
+
+    def step():
+        s.eval_dev(n, w.data_ptr(), x.data_ptr(), out.data_ptr(), dx.data_ptr(),
+                   phi.data_ptr())
+
+    for _ in range(warmup):
+        step()
+    s.sync()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        step()
+        ev[k][1].record(stream)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    res = dict(precision=precision, wall_s=wall, ms_per_step=wall * 1e3 / steps,
+               evals_per_s=steps / wall, p50_ms=float(np.percentile(lat, 50)),
+               p99_ms=float(np.percentile(lat, 99)), mean_event_ms=float(lat.mean()))
+    res["E"] = float(out[0].item())
+    res["vir"] = out[1:7].cpu().numpy().tolist()
+    res["dx"] = dx.cpu().numpy()
+    res["phi"] = phi.cpu().numpy()
+    if profile:
+        s.profile(True)
+        for _ in range(steps):
+            step()
+        prof = s.profile_read()
+        s.profile(False)
+        res["kernels"] = {k: {"total_ms": v[0], "launches": v[1],
+                              "avg_us": 1e3 * v[0] / max(v[1], 1)} for k, v in prof.items()}
+    del s
+    return res
+
+
+def cpu_baseline(sy, budget_s=15.0):
+    """The FP64 oracle (numpy restatement of the reference algorithm) timed
+    on this host, one thread, on a bounded number of evaluations."""
+    from oracle.sfac_oracle import OracleSfac, ewald_rad
+    o = OracleSfac(sy.L6, sy.K, sy.order)
+    o.set_A(ewald_rad(sy.beta), sy.max_m)
+    o.evaluate(sy.w, sy.x)  # warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.evaluate(sy.w, sy.x)
+        k += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"{sy.name}: {k} full evaluations (FP64 numpy oracle, "
+                      f"single thread, {dt:.1f} s; set_A excluded)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="water_98k")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "frames and requests shard across GPUs with no exchange, which is not "
+              "measured here.", file=sys.stderr)
+        sys.exit(2)
+
+    sy = workload(args.workload)
+    log(f"workload {sy.name}: n={sy.n} K={sy.K} order={sy.order} beta={sy.beta}")
+    r64 = time_engine(sy, "fp64", args.steps, args.warmup, profile=True)
+    log(f"fp64: {r64['evals_per_s']:.1f} evals/s  p50 {r64['p50_ms']:.3f} ms  "
+        f"p99 {r64['p99_ms']:.3f} ms")
+    r32 = time_engine(sy, "fp32", args.steps, args.warmup, profile=True)
+    log(f"fp32: {r32['evals_per_s']:.1f} evals/s  p50 {r32['p50_ms']:.3f} ms  "
+        f"p99 {r32['p99_ms']:.3f} ms")
+
+    # FP32 against the FP64 engine on the same inputs
+    eE = abs(r32["E"] - r64["E"]) / abs(r64["E"])
+    eF = float(np.sqrt(np.mean((r32["dx"] - r64["dx"]) ** 2)) /
+               np.sqrt(np.mean(r64["dx"] ** 2)))
+    ePhi = float(np.sqrt(np.mean((r32["phi"] - r64["phi"]) ** 2)) /
+                 np.sqrt(np.mean(r64["phi"] ** 2)))
+
+    # dominant kernel of the FP32 engine, timed live with HIP events
+    ks = r32["kernels"]
+    dom = max(ks, key=lambda k: ks[k]["total_ms"])
+    B = kernel_bytes(dom, sy.n, sy.K, 4, 8)
+    avg_s = ks[dom]["avg_us"] * 1e-6
+    achieved = B / avg_s / 1e9 if B else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{sy.name}_fp32.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "kernel": dom, "algorithmic_bytes": B, "avg_us": ks[dom]["avg_us"]}
+    Be32 = eval_bytes(sy.n, sy.K, 4, 8)
+    Be64 = eval_bytes(sy.n, sy.K, 8, 8)
+
+    cpu = None if args.no_cpu else cpu_baseline(sy, args.cpu_budget)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3f} evals/s ({cpu['sample']})")
+
+    line = {
+        "metric": METRIC,
+        "value": r32["evals_per_s"],
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": sy.name, "atoms": sy.n, "grid": list(sy.K), "order": sy.order,
+                   "beta": sy.beta, "max_m": sy.max_m, "precision": "fp32 (FP64 table, "
+                   "reductions and int64 fixed-point grid accumulation)"},
+        "latency_ms": {"p50": r32["p50_ms"], "p99": r32["p99_ms"]},
+        "fp64": {"value": r64["evals_per_s"], "ms_per_step": r64["ms_per_step"],
+                 "latency_ms": {"p50": r64["p50_ms"], "p99": r64["p99_ms"]},
+                 "dtype": "f64",
+                 "eval_roofline_frac": Be64 / (r64["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "speedup_fp32_vs_fp64": r32["evals_per_s"] / r64["evals_per_s"],
+        "fp32_error_vs_fp64_engine": {"energy_rel": eE, "force_rel_rms": eF,
+                                      "potential_rel_rms": ePhi},
+        "eval_roofline": {"algorithmic_bytes": Be32,
+                          "achieved_GBs": Be32 / (r32["mean_event_ms"] * 1e-3) / 1e9,
+                          "frac": Be32 / (r32["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "roofline": roofline,
+        "kernels_fp32_avg_us": {k: round(v["avg_us"], 3) for k, v in ks.items()},
+        "kernels_fp64_avg_us": {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,322 @@
+// libsfac.so C ABI (sfac.hpp:194-211 of the reference) and the SFac facade.
+//
+// Every entry point catches C++ exceptions: nothing is thrown across the
+// ABI.  Misuse that the reference reports with printf (sfac.cpp:310-317,
+// 348-355, 414-417, 502-505) prints the same message and returns 0.0 / void;
+// the message is also kept for sfac_last_error().  A failed constructor
+// returns NULL (the reference throws, which aborts a ctypes caller).
+#include <stdio.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ball.hpp"
+#include "engine.hpp"
+#include "sfac.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+
+void set_error(const std::string &m) { g_last_error = m; }
+
+SFac *as_sfac(void *h) {
+    if (!h) throw std::invalid_argument("NULL sfac handle");
+    return reinterpret_cast<SFac *>(h);
+}
+
+template <typename F>
+int guard(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return 0;
+    } catch (const spme::UsageError &e) {
+        printf("Error: %s\n", e.what());
+        fflush(stdout);
+        set_error(e.what());
+        return SFAC_ERR_USAGE;
+    } catch (const std::invalid_argument &e) {
+        set_error(e.what());
+        return SFAC_ERR_ARG;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return SFAC_ERR_DEVICE;
+    }
+}
+}  // namespace
+
+// ---- SFac facade ---------------------------------------------------------------
+SFac::SFac(double L[6], int32_t K_[3], int order_, int flags)
+    : order(order_), cell(L), state(-1), bspc(order_ > 0 ? order_ : 2), fft_neg(nullptr),
+      fft_pos(nullptr), A(nullptr), dA(nullptr), Q(nullptr), engine(nullptr) {
+    if (order % 2 != 0) {
+        printf("Bad order: %d (must be even)\n", order);
+        fflush(stdout);
+        throw std::invalid_argument("invalid order");
+    }
+    if (K_[0] < 1 || K_[1] < 1 || K_[2] < 1) {
+        printf("Bad K: %d %d %d\n", K_[0], K_[1], K_[2]);
+        fflush(stdout);
+        throw std::invalid_argument("invalid K");
+    }
+    for (int d = 0; d < 3; ++d) K[d] = K_[d];
+    ldim = K[2] / 2 + 1;
+    iK = 1.0 / ((double)K[0] * K[1] * K[2]);
+    int Ki[3] = {K[0], K[1], K[2]};
+    engine = new spme::Engine(L, Ki, order, flags);
+    for (int d = 0; d < 3; ++d) iB.push_back(engine->iB[d]);
+}
+
+SFac::~SFac() {
+    delete engine;
+    free(A);
+}
+
+std::multimap<int, Vec4> SFac::sort(int n, const double *w, const double *x) const {
+    std::multimap<int, Vec4> srt;
+    for (int a = 0; a < n; ++a) {
+        Vec4 u = cell.scale(x + 3 * a);
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            u[d] *= K[d];
+            c[d] = (int)u[d];
+            u[d] -= c[d];
+            if (c[d] >= K[d]) c[d] -= K[d];  // s == 1.0 -> periodic image
+        }
+        u[3] = w[a];
+        srt.insert(std::make_pair((c[0] * K[1] + c[1]) * K[2] + c[2], u));
+    }
+    return srt;
+}
+
+void SFac::operator()(int n, const double *w, const double *x) {
+    engine->sfac_host(n, w, x);
+    state = engine->state;
+}
+
+// ---- the reference C ABI -------------------------------------------------------------
+extern "C" {
+
+void *sfac_ctor_ex(const double L[6], const int K[3], int order, int flags) {
+    SFac *s = nullptr;
+    int rc = guard([&] {
+        double L6[6];
+        int32_t Ki[3] = {K[0], K[1], K[2]};
+        for (int i = 0; i < 6; ++i) L6[i] = L[i];
+        s = new SFac(L6, Ki, order, flags);
+    });
+    return rc == 0 ? reinterpret_cast<void *>(s) : nullptr;
+}
+
+void *sfac_ctor(double L[6], int K[3], int order) { return sfac_ctor_ex(L, K, order, 0); }
+
+void sfac_dtor(void *h) {
+    guard([&] { delete as_sfac(h); });
+}
+
+void sfac(void *h, int n, const double *w, const double *x) {
+    guard([&] { (*as_sfac(h))(n, w, x); });
+}
+
+double *get_S(void *h) {
+    double *p = nullptr;
+    guard([&] {
+        SFac *s = as_sfac(h);
+        p = s->engine->get_S_host();
+        s->Q = p;
+    });
+    return p;
+}
+
+void set_L(void *h, double L[6]) {
+    guard([&] {
+        SFac *s = as_sfac(h);
+        s->cell = Cell(L);
+        s->engine->set_cell(L);
+    });
+}
+
+// The reference enumerates the ball and calls f once per lattice point
+// (sfac.cpp:213-292); the table is assembled on the host in the same order
+// and uploaded once.
+int set_A(void *h, rad_fn f, double max_m, void *info) {
+    int status = 0;
+    int rc = guard([&] {
+        SFac *s = as_sfac(h);
+        const size_t H = (size_t)s->K[0] * s->K[1] * s->ldim;
+        if (!s->A) {
+            s->A = (double *)malloc(sizeof(double) * H * 7);
+            if (!s->A) {
+                status = 2;
+                return;
+            }
+            s->dA = s->A + H;
+        }
+        memset(s->A, 0, sizeof(double) * H * 7);
+        double *A = s->A, *dA = s->dA;
+        spme::enumerate_ball(s->cell.iL, s->K, s->ldim, max_m,
+                             [&](long idx, double m2, double mx, double my, double mz) {
+                                 double d = 0.0;
+                                 A[idx] += 0.5 * f(m2, &d, info);
+                                 double *p = dA + 6 * idx;
+                                 p[0] += d * mx * mx;
+                                 p[1] += d * my * my;
+                                 p[2] += d * mz * mz;
+                                 p[3] += d * my * mx;
+                                 p[4] += d * mz * mx;
+                                 p[5] += d * mz * my;
+                             });
+        s->engine->upload_table(A, dA);
+    });
+    if (rc != 0) return rc == SFAC_ERR_DEVICE ? 2 : rc;
+    return status;
+}
+
+double en(void *h) {
+    double e = 0.0;
+    guard([&] { e = as_sfac(h)->engine->en_host(); });
+    return e;
+}
+
+double de1(void *h, double vir[6]) {
+    double e = 0.0;
+    guard([&] {
+        SFac *s = as_sfac(h);
+        e = s->engine->de1_host(vir);
+        s->state = s->engine->state;
+    });
+    return e;
+}
+
+void de2(void *h, int n, const double *w, const double *x, double *dx) {
+    guard([&] { as_sfac(h)->engine->gather_host(n, w, x, dx, nullptr); });
+}
+
+void potl(void *h, int n, const double *x, double *phi) {
+    guard([&] { as_sfac(h)->engine->gather_host(n, nullptr, x, nullptr, phi); });
+}
+
+// ---- extensions (sfac_ext.h) -------------------------------------------------------
+const char *sfac_last_error(void) { return g_last_error.c_str(); }
+
+int sfac_set_A_ewald(void *h, double eta, double max_m) {
+    return guard([&] {
+        if (!(eta > 0.0)) throw std::invalid_argument("eta must be > 0");
+        SFac *s = as_sfac(h);
+        s->engine->build_ewald_table(eta, max_m);
+    });
+}
+
+int sfac_dev(void *h, int n, const double *w_d, const double *x_d) {
+    return guard([&] {
+        if (n < 0) throw std::invalid_argument("n < 0");
+        SFac *s = as_sfac(h);
+        s->engine->spread_forward(n, w_d, x_d);
+        s->state = s->engine->state;
+    });
+}
+
+int en_dev(void *h, double *out_d) {
+    return guard([&] { as_sfac(h)->engine->energy(out_d); });
+}
+
+int de1_dev(void *h, double *out_d) {
+    return guard([&] {
+        SFac *s = as_sfac(h);
+        s->engine->convolve(out_d);
+        s->state = s->engine->state;
+    });
+}
+
+int de2_dev(void *h, int n, const double *w_d, const double *x_d, double *dx_d) {
+    return guard([&] { as_sfac(h)->engine->gather(n, w_d, x_d, dx_d, nullptr); });
+}
+
+int potl_dev(void *h, int n, const double *x_d, double *phi_d) {
+    return guard([&] { as_sfac(h)->engine->gather(n, nullptr, x_d, nullptr, phi_d); });
+}
+
+int sfac_eval_dev(void *h, int n, const double *w_d, const double *x_d, double *out_d,
+                  double *dx_d, double *phi_d) {
+    return guard([&] {
+        if (n < 0) throw std::invalid_argument("n < 0");
+        SFac *s = as_sfac(h);
+        spme::Engine *e = s->engine;
+        if (!e->has_table()) throw spme::UsageError("No convolution kernel has been set.");
+        e->spread_forward(n, w_d, x_d);
+        e->convolve(out_d ? out_d : e->result_dev());
+        e->gather(n, w_d, x_d, dx_d, phi_d);
+        s->state = e->state;
+    });
+}
+
+double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6], double *dx,
+                 double *phi) {
+    double E = 0.0;
+    guard([&] {
+        SFac *s = as_sfac(h);
+        s->engine->sfac_host(n, w, x);
+        double v[6];
+        E = s->engine->de1_host(v);
+        if (vir)
+            for (int c = 0; c < 6; ++c) vir[c] = v[c];
+        s->engine->gather_host(n, w, x, dx, phi);
+        s->state = s->engine->state;
+    });
+    return E;
+}
+
+void *sfac_stream(void *h) {
+    void *st = nullptr;
+    guard([&] { st = reinterpret_cast<void *>(as_sfac(h)->engine->stream()); });
+    return st;
+}
+
+int sfac_sync(void *h) {
+    return guard([&] { as_sfac(h)->engine->sync(); });
+}
+
+int sfac_info(void *h, long long out[8]) {
+    return guard([&] {
+        spme::Engine *e = as_sfac(h)->engine;
+        out[0] = e->K[0];
+        out[1] = e->K[1];
+        out[2] = e->K[2];
+        out[3] = e->ldim;
+        out[4] = e->order;
+        out[5] = (e->fp32 ? SFAC_FP32 : 0) | (e->fixed ? SFAC_FIXED : 0);
+        out[6] = (long long)e->device_bytes();
+        out[7] = e->state;
+    });
+}
+
+int sfac_profile(void *h, int enable) {
+    return guard([&] { as_sfac(h)->engine->profile_enable(enable != 0); });
+}
+
+int sfac_profile_read(void *h, char *names, int names_len, double *ms, int *counts, int max) {
+    int n = 0;
+    int rc = guard([&] {
+        std::vector<std::string> nm;
+        std::vector<double> t;
+        std::vector<int> c;
+        as_sfac(h)->engine->profile_read(nm, t, c);
+        std::string joined;
+        for (size_t k = 0; k < nm.size() && (int)k < max; ++k) {
+            if (k) joined += "\n";
+            joined += nm[k];
+            ms[k] = t[k];
+            counts[k] = c[k];
+            ++n;
+        }
+        if (names_len > 0) {
+            strncpy(names, joined.c_str(), (size_t)names_len - 1);
+            names[names_len - 1] = '\0';
+        }
+    });
+    return rc == 0 ? n : rc;
+}
+
+}  // extern "C"

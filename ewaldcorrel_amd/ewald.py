@@ -1,0 +1,309 @@
+#!/usr/bin/env python3
+"""Python entry points of the SPME engine -- a drop-in for the reference's
+ewald.py (frobnitzem/EwaldCorrel ewald.py:1-141).
+
+Same names and call shapes: ``describe_sfac``, ``Sfac(L, K, order=4)`` with
+``.sfac(w, x)``, ``.get_S()``, ``.set_A(f, max_m)``, ``.set_L(L)`` and the
+forwarded ``.en()``, ``.de1(vir)``, ``.de2(N, w, x, dx)``, ``.potl(N, x,
+phi)``; module-level ``eta``, ``ewald_f``, ``LofS``, ``SofL``, ``array``.
+``libsfac.so`` is loaded from this file's directory (ewald.py:52-53); it is
+the HIP library, and there is no fallback when it is missing.
+
+Additions (keyword arguments and new methods only):
+  * ``Sfac(..., precision="fp64"|"fp32", fixed=False)``
+  * ``set_A`` recognises the closed-form Ewald kernel (``ewald_f`` or a
+    kernel from ``ewald_kernel(eta)``) and builds the table natively,
+    instead of one Python callback per lattice point
+  * ``evaluate(w, x)`` -> (E, vir, dx, phi): one evaluation in one call
+  * ``*_dev`` methods taking device pointers (ints), asynchronous on
+    ``stream``; ``sync()``
+"""
+import os
+from functools import reduce
+from math import pi, exp, erfc  # noqa: F401  (reference module namespace)
+
+from ctypes import CDLL, c_void_p, c_int, c_double, c_char_p, c_longlong, \
+    POINTER, PYFUNCTYPE, cast
+
+import numpy as np
+import numpy.ctypeslib as ct
+
+array = np.array
+
+SFAC_FP64 = 0
+SFAC_FP32 = 1
+SFAC_FIXED = 2
+
+_LIB = None
+
+
+def _libpath():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsfac.so")
+
+
+def describe_sfac():
+    """Load libsfac.so and declare the C signatures (ewald.py:15-68).
+
+    Returns (sfac_p, rad_fn, lib) like the reference."""
+    global _LIB
+
+    def decl_fn(a, *args):
+        a.argtypes = args[:-1]
+        a.restype = args[-1]
+
+    def void_fn(a, *args):
+        decl_fn(a, *(args + (None,)))
+
+    def int_fn(a, *args):
+        decl_fn(a, *(args + (c_int,)))
+
+    def dbl_fn(a, *args):
+        decl_fn(a, *(args + (c_double,)))
+
+    nparr = np.ctypeslib.ndpointer(dtype=np.float64, flags='C_CONTIGUOUS')
+    dblarr = POINTER(c_double)
+
+    def nparr_t(*shape):
+        return np.ctypeslib.ndpointer(dtype=np.float64, shape=shape, flags='C_CONTIGUOUS')
+
+    def np_intarr_t(*shape):
+        return np.ctypeslib.ndpointer(dtype=np.int32, shape=shape, flags='C_CONTIGUOUS')
+
+    sfac_p = c_void_p
+    rad_fn = PYFUNCTYPE(c_double, c_double, POINTER(c_double), c_void_p)
+
+    if _LIB is None:
+        path = _libpath()
+        if not os.path.exists(path):
+            raise ImportError(
+                "libsfac.so (the HIP SPME library) is not built; run "
+                "`python -m ewaldcorrel_amd._build` (expected at %s)" % path)
+        lib = CDLL(path)
+        decl_fn(lib.sfac_ctor, nparr_t(6), np_intarr_t(3), c_int, sfac_p)
+        void_fn(lib.sfac_dtor, sfac_p)
+        void_fn(lib.sfac, sfac_p, c_int, nparr, nparr)
+        decl_fn(lib.get_S, sfac_p, dblarr)
+        void_fn(lib.set_L, sfac_p, nparr_t(6))
+        int_fn(lib.set_A, sfac_p, rad_fn, c_double, c_void_p)
+        dbl_fn(lib.en, sfac_p)
+        dbl_fn(lib.de1, sfac_p, nparr_t(6))
+        void_fn(lib.de2, sfac_p, c_int, nparr, nparr, nparr)
+        void_fn(lib.potl, sfac_p, c_int, nparr, nparr)
+        # extensions (sfac_ext.h)
+        decl_fn(lib.sfac_ctor_ex, nparr_t(6), np_intarr_t(3), c_int, c_int, sfac_p)
+        decl_fn(lib.sfac_last_error, c_char_p)
+        int_fn(lib.sfac_set_A_ewald, sfac_p, c_double, c_double)
+        int_fn(lib.sfac_dev, sfac_p, c_int, c_void_p, c_void_p)
+        int_fn(lib.en_dev, sfac_p, c_void_p)
+        int_fn(lib.de1_dev, sfac_p, c_void_p)
+        int_fn(lib.de2_dev, sfac_p, c_int, c_void_p, c_void_p, c_void_p)
+        int_fn(lib.potl_dev, sfac_p, c_int, c_void_p, c_void_p)
+        int_fn(lib.sfac_eval_dev, sfac_p, c_int, c_void_p, c_void_p, c_void_p,
+               c_void_p, c_void_p)
+        dbl_fn(lib.sfac_eval, sfac_p, c_int, nparr, nparr, nparr_t(6), nparr, nparr)
+        decl_fn(lib.sfac_stream, sfac_p, c_void_p)
+        int_fn(lib.sfac_sync, sfac_p)
+        int_fn(lib.sfac_info, sfac_p, POINTER(c_longlong))
+        int_fn(lib.sfac_profile, sfac_p, c_int)
+        int_fn(lib.sfac_profile_read, sfac_p, c_char_p, c_int, POINTER(c_double),
+               POINTER(c_int), c_int)
+        _LIB = lib
+    return sfac_p, rad_fn, _LIB
+
+
+def last_error():
+    _, _, lib = describe_sfac()
+    return lib.sfac_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+class Sfac:
+    """Handle on one SPME system (ewald.py:70-122)."""
+
+    def __init__(self, L, K, order=4, precision="fp64", fixed=False):
+        sfac_p, rad_fn, libsfac = describe_sfac()
+        self._libsfac = libsfac
+        self.rad_fn = rad_fn
+        K = np.asarray(K)
+        self.K = tuple(K.tolist())
+        self.ldim = K[2] // 2 + 1
+        flags = 0
+        if precision == "fp32":
+            flags |= SFAC_FP32
+        elif precision != "fp64":
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if fixed:
+            flags |= SFAC_FIXED
+        self.precision = precision
+        L = np.ascontiguousarray(np.asarray(L, dtype=np.float64))
+        self._data = libsfac.sfac_ctor_ex(L, np.ascontiguousarray(K.astype(np.int32)),
+                                          order, flags)
+        if not self._data:
+            raise RuntimeError("Error intializing Sfac. (%s)" % last_error())
+
+    def __getattr__(self, name):
+        """Pass the handle as the first argument to the forwarded C
+        functions (ewald.py:83-95)."""
+        names = ["sfac_dtor", "en", "de1", "de2", "potl"]
+        if name not in names:
+            raise AttributeError(name)
+        try:
+            f = getattr(self._libsfac, name)
+        except AttributeError:
+            return None
+        return lambda *args, **kws: f(self._data, *args, **kws)
+
+    def get_S(self):
+        """Complex (K0, K1, ldim) copy of the grid (ewald.py:97-101)."""
+        Q = self._libsfac.get_S(self._data)
+        if not Q:
+            raise RuntimeError("get_S failed: %s" % last_error())
+        shape = (self.K[0], self.K[1], self.ldim, 2)
+        s = ct.as_array(Q, shape=shape)
+        return s[:, :, :, 0] + 1j * s[:, :, :, 1]
+
+    def sfac(self, w, x):
+        assert len(w.shape) == 1
+        assert x.shape == (len(w), 3)
+        self._libsfac.sfac(self._data, len(w), np.ascontiguousarray(w, dtype=np.float64),
+                           np.ascontiguousarray(x, dtype=np.float64))
+
+    def __del__(self):
+        if getattr(self, "_data", None):
+            self._libsfac.sfac_dtor(self._data)
+            self._data = None
+
+    def set_A(self, f, max_m):
+        self.f = f
+        eta_native = _native_eta(f)
+        if eta_native is not None:
+            n = self._libsfac.sfac_set_A_ewald(self._data, eta_native, max_m)
+        else:
+            n = self._libsfac.set_A(self._data, self.rad_fn(f), max_m, cast(0, c_void_p))
+        if n != 0:
+            raise RuntimeError("Error (%d) setting A-array." % n)
+
+    def set_L(self, L):
+        self._libsfac.set_L(self._data, np.ascontiguousarray(L, dtype=np.float64))
+        if hasattr(self, "f"):
+            self.set_A(self.f, 1.0)
+
+    # ---- additions -------------------------------------------------------
+    def evaluate(self, w, x):
+        """One evaluation (sfac + de1 + de2 + potl); returns E, vir, dx, phi."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = len(w)
+        vir = np.zeros(6)
+        dx = np.zeros((n, 3))
+        phi = np.zeros(n)
+        E = self._libsfac.sfac_eval(self._data, n, w, x, vir, dx, phi)
+        err = last_error()
+        if err:
+            raise RuntimeError("evaluate failed: " + err)
+        return E, vir, dx, phi
+
+    @property
+    def stream(self):
+        return self._libsfac.sfac_stream(self._data)
+
+    def sync(self):
+        _check(self._libsfac.sfac_sync(self._data), "sync")
+
+    def sfac_dev(self, n, w_ptr, x_ptr):
+        _check(self._libsfac.sfac_dev(self._data, n, w_ptr, x_ptr), "sfac_dev")
+
+    def en_dev(self, out_ptr):
+        _check(self._libsfac.en_dev(self._data, out_ptr), "en_dev")
+
+    def de1_dev(self, out_ptr):
+        _check(self._libsfac.de1_dev(self._data, out_ptr), "de1_dev")
+
+    def de2_dev(self, n, w_ptr, x_ptr, dx_ptr):
+        _check(self._libsfac.de2_dev(self._data, n, w_ptr, x_ptr, dx_ptr), "de2_dev")
+
+    def potl_dev(self, n, x_ptr, phi_ptr):
+        _check(self._libsfac.potl_dev(self._data, n, x_ptr, phi_ptr), "potl_dev")
+
+    def eval_dev(self, n, w_ptr, x_ptr, out_ptr, dx_ptr, phi_ptr):
+        _check(self._libsfac.sfac_eval_dev(self._data, n, w_ptr, x_ptr, out_ptr,
+                                           dx_ptr, phi_ptr), "sfac_eval_dev")
+
+    def profile(self, enable=True):
+        """Bracket every kernel launch with HIP events on ``stream``."""
+        _check(self._libsfac.sfac_profile(self._data, int(bool(enable))), "profile")
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} recorded since profiling was enabled."""
+        import ctypes
+        mx = 64
+        names = ctypes.create_string_buffer(4096)
+        ms = (c_double * mx)()
+        cnt = (c_int * mx)()
+        n = self._libsfac.sfac_profile_read(self._data, names, 4096, ms, cnt, mx)
+        if n < 0:
+            raise RuntimeError("profile_read failed: " + last_error())
+        keys = names.value.decode().split("\n") if n else []
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+    def info(self):
+        out = (c_longlong * 8)()
+        _check(self._libsfac.sfac_info(self._data, out), "info")
+        keys = ("K0", "K1", "K2", "ldim", "order", "flags", "device_bytes", "state")
+        return dict(zip(keys, list(out)))
+
+
+# ---- the Ewald kernel (ewald.py:124-132) ------------------------------------
+eta = 0.5
+_fac = -(pi / eta) ** 2
+
+
+def ewald_f(r2, ptr, info):
+    """f(r2) = exp(-pi^2 r2/eta^2)/(pi r2); *ptr = f'(r2); 0 for r2 < 1e-10.
+
+    Like the reference it reads the module-level ``_fac`` (set from ``eta``
+    at import).  ``Sfac.set_A`` recognises this function and builds the same
+    table natively with eta = pi/sqrt(-_fac)."""
+    if r2 < 1e-10:
+        ptr.contents.value = 0.0
+        return 0.0
+    f = exp(r2 * _fac) / (r2 * pi)
+    ptr.contents.value = (_fac - 1. / r2) * f
+    return f
+
+
+def ewald_kernel(eta_):
+    """rad_fn-compatible Ewald kernel with its own eta (e.g. eta = beta);
+    recognised by ``Sfac.set_A`` for the native table build."""
+    fac = -(pi / eta_) ** 2
+
+    def f(r2, ptr, info):
+        if r2 < 1e-10:
+            ptr.contents.value = 0.0
+            return 0.0
+        v = exp(r2 * fac) / (r2 * pi)
+        ptr.contents.value = (fac - 1. / r2) * v
+        return v
+
+    f._ewald_eta = float(eta_)
+    return f
+
+
+def _native_eta(f):
+    if f is ewald_f:
+        return pi / (-_fac) ** 0.5
+    return getattr(f, "_ewald_eta", None)
+
+
+def LofS(L):
+    return np.array([[L[0], 0., 0.],
+                     [L[3], L[1], 0.],
+                     [L[4], L[5], L[2]]])
+
+
+def SofL(L):
+    return np.array([L[0, 0], L[1, 1], L[2, 2], L[1, 0], L[2, 0], L[2, 1]])

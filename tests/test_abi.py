@@ -1,0 +1,87 @@
+"""The C-ABI library loads and exports every symbol the public headers
+declare; argument validation that needs no device.  CPU only (no compute
+call reaches the GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+LIB = os.path.join(ROOT, "ewaldcorrel_amd", "libsfac.so")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("sfac.hpp", "sfac_ext.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        # only the extern "C" blocks
+        for block in re.findall(r'extern "C" \{(.*?)\n\}', text, re.S):
+            block = re.sub(r"//[^\n]*|/\*.*?\*/", "", block, flags=re.S)
+            block = re.sub(r"^\s*#[^\n]*", "", block, flags=re.M)
+            for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", block):
+                name = m.group(1)
+                if name not in ("typedef", "double", "int", "void", "const"):
+                    names.add(name)
+    names.discard("rad_fn")
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libsfac.so is not built (run __graft_entry__.build())")
+    return ctypes.CDLL(LIB)
+
+
+def test_headers_declare_reference_abi():
+    ref = {"sfac_ctor", "sfac_dtor", "sfac", "get_S", "set_L", "set_A", "en", "de1", "de2",
+           "potl"}
+    assert ref <= declared_functions()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_declares_everything():
+    from ewaldcorrel_amd import ewald
+    sfac_p, rad_fn, lib = ewald.describe_sfac()
+    for n in declared_functions():
+        assert getattr(lib, n).restype is not None or n in (
+            "sfac_dtor", "sfac", "set_L", "de2", "potl"), n
+
+
+def test_ctor_rejects_bad_arguments_without_device():
+    """Odd order and K < 1 are rejected before any device call (the
+    reference prints and throws, sfac.cpp:41-51); here the ctor returns NULL
+    and the binding raises."""
+    from ewaldcorrel_amd import ewald
+    L = np.array([10., 10., 10., 0., 0., 0.])
+    with pytest.raises(RuntimeError, match="invalid order"):
+        ewald.Sfac(L, np.array([8, 8, 8]), 3)
+    with pytest.raises(RuntimeError, match="invalid K"):
+        ewald.Sfac(L, np.array([8, 0, 8]), 4)
+    with pytest.raises(ValueError):
+        ewald.Sfac(L, np.array([8, 8, 8]), 4, precision="fp16")
+
+
+def test_null_handle_is_an_error_not_a_crash(lib):
+    lib.en.restype = ctypes.c_double
+    lib.en.argtypes = [ctypes.c_void_p]
+    assert lib.en(None) == 0.0
+    lib.sfac_last_error.restype = ctypes.c_char_p
+    assert b"NULL" in lib.sfac_last_error()
+
+
+def test_oracle_not_linked_into_product():
+    """The product library and binding never reference the oracle."""
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "ewaldcorrel_amd")):
+        for fn in files:
+            if fn.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, fn)).read()
+                assert "oracle" not in text.lower(), fn
