@@ -46,25 +46,23 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def kernel_bytes(kernel, n, K, p, pg):
+def kernel_bytes(kernel, n, K, p):
     """Algorithmic HBM bytes of one launch: the compulsory reads and writes of
-    that kernel.  p = working-precision bytes, pg = grid-accumulation bytes;
-    positions/charges/forces/potentials are doubles at the ABI (8 B)."""
+    that kernel.  p = working-precision bytes of the grid; positions,
+    charges, forces and potentials are doubles at the ABI (8 B each)."""
     K0, K1, K2 = K
     G = K0 * K1 * K2
     H = K0 * K1 * (K2 // 2 + 1)
     table = {
-        "spread": n * 4 * 8 + G * pg,
-        "fixed_to_grid": G * pg + G * p,
-        "zero_grid": 2 * H * p,
+        "prep": n * 4 * 8 + n * 8,             # x, w in; column, rank out
+        "scatter": n * (4 * 8 + 8) + n * (3 * p + 16),  # x, w, col, rank in; records out
+        "spread": n * (3 * p + 16) + G * p,    # column-sorted records in, grid out once
         "fft_z_r2c": G * p + 2 * H * p,
         "fft_y_fwd": 4 * H * p,
-        "fft_x_fwd": 4 * H * p,
-        "influence": 2 * H * p + 2 * H * p + 8 * H,
-        "fft_x_inv": 4 * H * p,
+        "fft_x_recip": 4 * H * p,              # closed-form influence: no table read
         "fft_y_inv": 4 * H * p,
         "fft_z_c2r": 2 * H * p + G * p,
-        "gather": G * p + n * 4 * 8 + n * 4 * 8,
+        "gather_sorted": G * p + n * (4 * 8 + 4) + n * 4 * 8,
     }
     return table.get(kernel)
 
@@ -190,7 +188,7 @@ def main():
     # dominant kernel of the FP32 engine, timed live with HIP events
     ks = r32["kernels"]
     dom = max(ks, key=lambda k: ks[k]["total_ms"])
-    B = kernel_bytes(dom, sy.n, sy.K, 4, 8)
+    B = kernel_bytes(dom, sy.n, sy.K, 4)
     avg_s = ks[dom]["avg_us"] * 1e-6
     achieved = B / avg_s / 1e9 if B else None
     traffic = None
@@ -201,7 +199,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "kernel": dom, "algorithmic_bytes": B, "avg_us": ks[dom]["avg_us"]}
-    Be32 = eval_bytes(sy.n, sy.K, 4, 8)
+    Be32 = eval_bytes(sy.n, sy.K, 4, 4)  # the FP32 grid is written once, as FP32
     Be64 = eval_bytes(sy.n, sy.K, 8, 8)
 
     cpu = None if args.no_cpu else cpu_baseline(sy, args.cpu_budget)

@@ -4,6 +4,11 @@
 // sfac.cpp:70-74, so get_S is a plain copy): a K0 x K1 x (2*ldim) array of
 // reals with K2 of each row used; the same memory viewed as K0 x K1 x ldim
 // interleaved complex numbers holds the half-complex spectrum.
+//
+// Every kernel receives FrameDescs (one per SPME system: grid, box, tables,
+// atoms, workspace) in device memory.  A single evaluation passes one; a
+// batch passes the array plus, per stage, the prefix sums of the blocks each
+// frame needs, and every workgroup finds its frame by binary search.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,6 +18,9 @@ namespace spme {
 
 constexpr int kMaxOrder = 12;      // largest even B-spline order compiled in
 constexpr int kMaxRadixStages = 16;
+constexpr int kBlock = 256;        // threads per workgroup (spreading uses kSpreadBlock)
+constexpr int kSpreadBlock = 1024;
+constexpr int kMaxK = 1024;        // largest grid dimension supported
 
 template <typename T> struct cplx_t;
 template <> struct cplx_t<float> { using type = float2; };
@@ -26,16 +34,86 @@ struct CellArgs {
     double iV;
 };
 
-struct GridArgs {
-    int K0, K1, K2;
-    int ldim;      // K2/2 + 1 complex numbers per row
-    int order;
+struct FftPlanArgs {
+    int N;
+    int nstages;
+    int generic;  // a stage with a radix other than 2,3,4,5,8 (needs two buffers)
+    int radix[kMaxRadixStages];
 };
 
 // Coefficients of M_n(x+j), x in [0,1), row j, highest power first.
 template <typename T> struct SplineArgs {
     T M[kMaxOrder * kMaxOrder];
 };
+
+enum TableMode {
+    kTableFull = 0,      // A[H] and dA[6H] read from HBM (any rad_fn)
+    kTableAnalytic = 1,  // closed-form Ewald A and dA from the indices (no table)
+};
+
+struct FrameDesc {
+    // geometry
+    int K0, K1, K2, ldim, order;
+    int n;  // atoms
+    CellArgs cell;
+    // grid (T*) and FFT data
+    void *grid;
+    const double *iB0, *iB1, *iB2;
+    const void *tw0, *tw1, *tw2;  // cplx<T> twiddles exp(-2 pi i t/K_d)
+    FftPlanArgs plan0, plan1, plan2;
+    // influence function
+    const double *table;  // A[H] then dA[6H] (kTableFull)
+    int tabmode;
+    double ewald_fac;  // -(pi/eta)^2
+    double max_m2;
+    CellArgs tcell;    // box the influence function was built for (set_L keeps it)
+    // atoms (device pointers, caller-owned)
+    const double *w, *x;
+    double *dx, *phi;
+    double *out;  // E, virial[6]
+    // column sort + spreading workspace
+    int *colcount;   // K0*K1 counters, zero between evaluations
+    int *colstart;   // K0*K1 + 1 exclusive prefix
+    int *a_col, *a_rank;
+    void *s_frac;    // T[3n], fractional offsets in column order
+    double *s_w;
+    int *s_cell;     // (cx*K1 + cy)*K2 + cz
+    int *s_perm;     // original atom index
+    unsigned long long *maxbits;  // max |w| as bits (fixed point)
+    double *fxscale;              // fixed-point scale, written by the scan
+    double mmax3;
+    int fixed;
+    int tx, ty, ntx, nty;  // spreading tiles
+    // FFT line blocking
+    int zC;          // complex lines (two real rows each) per z workgroup
+    int yC, yT;      // lines per y tile, tiles over ldim
+    int xC, xT;      // lines per x tile, tiles over ldim
+    // reductions (influence)
+    double *partials;
+    int nred_i;      // workgroups of the standalone influence launch
+};
+
+// Kernel arguments of every stage: frames[0..nf) in device memory (read
+// only, so the uniform descriptor fields are fetched with scalar loads) and,
+// for nf > 1, offs[f] = first workgroup of frame f in this launch.
+#define SPME_FRAME_ARGS \
+    const FrameDesc *__restrict__ frames, const int *__restrict__ offs, int nf
+
+__device__ __forceinline__ int find_frame(const int *__restrict__ offs, int nf, int &b) {
+    const int bid = blockIdx.x;
+    if (nf <= 1) {
+        b = bid;
+        return 0;
+    }
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    b = bid - offs[lo];
+    return lo;
+}
 
 // Fractional coordinates wrapped to [0,1): s = wrap(iL^T x) (sfac.hpp:56-62).
 __host__ __device__ inline void scale_wrap(const CellArgs &c, double x0, double x1,

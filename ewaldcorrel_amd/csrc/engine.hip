@@ -1,22 +1,30 @@
-// SPME reciprocal-space engine: buffers, FFT plans and kernel sequencing.
+// SPME reciprocal-space engine: buffers, launch geometry and kernel sequencing.
 //
-// One evaluation on the device (sfac + de1 + de2 + potl):
-//   spread (FP64 atomics, or int64 fixed point + convert)
-//   fft_z_r2c -> fft_strided(y) -> fft_strided(x, folds 1/(b0 b1 b2))  = S-hat
-//   influence_kernel<APPLY> + reduce_finalize                          = E, virial
-//   fft_strided(x, +1) -> fft_strided(y, +1) -> fft_z_c2r              = conv grid
-//   gather_kernel                                                       = dx, phi
+// One evaluation (sfac + de1 + de2 + potl), ten launches, no memsets:
+//   prep -> scan -> scatter -> spread          column sort + LDS-tiled spreading
+//   fft_z_r2c -> fft_y_fwd                      forward transform, z and y
+//   fft_x_recip                                 x forward, /B, influence, E and
+//                                               virial, x inverse (one pass)
+//   fft_y_inv -> fft_z_c2r                      inverse transform, y and z
+//   gather_sorted                               forces and potentials
+// The split entry points (sfac / en / de1 / de2 / potl) run the same kernels
+// with the x pass, influence and inverse x pass as separate launches, so that
+// S-hat is in HBM between sfac and de1 as the reference's state machine
+// requires.
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <type_traits>
 
 #include "ball.hpp"
 #include "engine.hpp"
 #include "fft_kernels.hpp"
+#include "gather_kernels.hpp"
+#include "recip_kernels.hpp"
 #include "sfac.hpp"
-#include "spme_kernels.hpp"
+#include "spread_kernels.hpp"
 
 namespace spme {
 
@@ -27,7 +35,15 @@ void hip_check(hipError_t e, const char *what) {
     }
 }
 #define HIPC(x) hip_check((x), #x)
-#define KCHECK(name) hip_check(hipGetLastError(), name)
+
+const char *stage_name(Stage s) {
+    static const char *names[ST_COUNT] = {
+        "prep",      "scan",      "scatter",   "spread",    "fft_z_r2c",
+        "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
+        "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
+        "finish"};
+    return names[s];
+}
 
 namespace {
 
@@ -51,22 +67,27 @@ FftPlanArgs plan_args(const FftPlan &p) {
     memset(&a, 0, sizeof(a));
     a.N = p.N;
     a.nstages = (int)p.radix.size();
-    for (int s = 0; s < a.nstages; ++s) a.radix[s] = p.radix[s];
+    for (int s = 0; s < a.nstages; ++s) {
+        a.radix[s] = p.radix[s];
+        const int R = p.radix[s];
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) a.generic = 1;
+    }
     return a;
 }
 
-template <typename T> const cplx<T> *tw_of(const FftPlan &p);
-template <> const float2 *tw_of<float>(const FftPlan &p) { return p.tw_f; }
-template <> const double2 *tw_of<double>(const FftPlan &p) { return p.tw_d; }
-
-// lines per workgroup so that two ping-pong line buffers fit the budget
-template <typename T> int lines_per_block(int N, int want) {
-    const size_t per = 2 * (size_t)(N + 1) * sizeof(cplx<T>);
-    int c = (int)std::max<size_t>(1, kLdsBudget / per);
-    return std::max(1, std::min(c, want));
+size_t lds_fft(bool fp32, int N, int C, int generic, bool strided) {
+    if (strided)
+        return fp32 ? strided_lds_bytes<float>(N, C, generic)
+                    : strided_lds_bytes<double>(N, C, generic);
+    return fp32 ? fft_lds_bytes<float>(N, C, generic) : fft_lds_bytes<double>(N, C, generic);
 }
-template <typename T> size_t lds_bytes(int N, int C) {
-    return 2 * (size_t)(N + 1) * C * sizeof(cplx<T>);
+
+// lines per FFT workgroup: up to fft_elems<T>() elements, within the LDS budget
+int fft_lines(bool fp32, int N, int avail, int generic, bool strided) {
+    const int target = fp32 ? fft_elems<float>() : fft_elems<double>();
+    int C = std::max(1, std::min(target / std::max(N, 1), std::max(avail, 1)));
+    while (C > 1 && lds_fft(fp32, N, C, generic, strided) > kLdsBudget) --C;
+    return C;
 }
 
 template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, int order) {
@@ -76,31 +97,74 @@ template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, in
     return s;
 }
 
-GridArgs grid_args(const Engine &e) {
-    GridArgs g;
-    g.K0 = e.K[0];
-    g.K1 = e.K[1];
-    g.K2 = e.K[2];
-    g.ldim = e.ldim;
-    g.order = e.order;
-    return g;
+template <typename T, int ORDER>
+void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
+              hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
+    const dim3 g(blocks), bl(kBlock);
+    switch (s) {
+        case ST_PREP: prep_kernel<<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_SCAN: scan_kernel<<<g, 1024, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_SCATTER: scatter_kernel<T><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_SPREAD: {
+            const dim3 sb(kSpreadBlock);
+            if constexpr (std::is_same<T, float>::value) {
+                spread_tile_kernel<T, ORDER, true><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
+            } else {
+                if (fixed) spread_tile_kernel<T, ORDER, true><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
+                else spread_tile_kernel<T, ORDER, false><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
+            }
+            break;
+        }
+        case ST_ZR2C: fft_z_r2c_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_YF: fft_strided_kernel<T, 1, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0); break;
+        case ST_XF_SCALE:
+            fft_strided_kernel<T, 0, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0);
+            break;
+        case ST_XFUSED: fft_strided_kernel<T, 0, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0); break;
+        case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_XI:
+            fft_strided_kernel<T, 0, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,+1, finish);
+            break;
+        case ST_YI:
+            fft_strided_kernel<T, 1, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,+1, finish);
+            break;
+        case ST_FINISH: finish_kernel<<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_ZC2R: fft_z_c2r_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_GATHER: gather_kernel<T, ORDER, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf,sp); break;
+        case ST_GATHER_SORTED:
+            gather_kernel<T, ORDER, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf,sp);
+            break;
+        default: break;
+    }
+    hip_check(hipGetLastError(), stage_name(s));
 }
 
-__global__ void zero_kernel(unsigned long long *p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x)
-        p[i] = 0ull;
-}
-__global__ void zero_words(unsigned int *p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x)
-        p[i] = 0u;
+template <typename T>
+void launch_order(int order, Stage s, const Launch &L, int blocks,
+                  size_t lds, hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
+    switch (order) {
+        case 4: launch_t<T, 4>(s, L, blocks, lds, st, sp, fixed, finish); break;
+        case 6: launch_t<T, 6>(s, L, blocks, lds, st, sp, fixed, finish); break;
+        case 8: launch_t<T, 8>(s, L, blocks, lds, st, sp, fixed, finish); break;
+        default: launch_t<T, 0>(s, L, blocks, lds, st, sp, fixed, finish); break;
+    }
 }
 
-int grid_blocks(long work, int block, int cap = 4096) {
-    long b = (work + block - 1) / block;
-    return (int)std::max<long>(1, std::min<long>(b, cap));
+// the reduction of a stage is finalized by the next launch of the sequence:
+// 1 after the fused x pass, 2 after the standalone influence launch
+int finishes(Stage prev, Stage s) {
+    if (prev == ST_XFUSED && s == ST_YI) return 1;
+    if (prev == ST_INFL && s == ST_XI) return 2;
+    return 0;
 }
+
+const Stage kSfacStages[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD,
+                             ST_ZR2C, ST_YF,   ST_XF_SCALE};
+const Stage kDe1Stages[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
+const Stage kEnStages[] = {ST_INFL_E, ST_FINISH};
+const Stage kEvalStages[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C,
+                             ST_YF,   ST_XFUSED, ST_YI,    ST_ZC2R,   ST_GATHER_SORTED};
 
 }  // namespace
 
@@ -109,8 +173,14 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     if (order_ % 2 != 0 || order_ < 2 || order_ > kMaxOrder)
         throw std::invalid_argument("invalid order (even, 2.." + std::to_string(kMaxOrder) +
                                     "): " + std::to_string(order_));
-    if (K_[0] < 1 || K_[1] < 1 || K_[2] < 1) throw std::invalid_argument("invalid K");
-    for (int d = 0; d < 3; ++d) K[d] = K_[d];
+    for (int d = 0; d < 3; ++d) {
+        if (K_[d] < 1) throw std::invalid_argument("invalid K");
+        if (K_[d] > kMaxK)
+            throw std::invalid_argument("grid dimension " + std::to_string(K_[d]) +
+                                        " above the supported maximum " +
+                                        std::to_string(kMaxK));
+        K[d] = K_[d];
+    }
     order = order_;
     ldim = K[2] / 2 + 1;
     fp32 = (flags & kFlagFP32) != 0;
@@ -131,21 +201,19 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
             iB[d][i] = 1.0 / s;
         }
     }
+    sp_f_ = spline_args<float>(M, order);
+    sp_d_ = spline_args<double>(M, order);
+    memset(&desc_, 0, sizeof(desc_));
     set_cell(L6_);
+    desc_.tcell = desc_.cell;
 
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     const size_t tsz = fp32 ? sizeof(float) : sizeof(double);
     alloc_(&grid_, grid_reals() * tsz);
-    if (fixed) alloc_((void **)&acc_, G() * sizeof(unsigned long long));
     for (int d = 0; d < 3; ++d) {
         alloc_((void **)&iB_d_[d], K[d] * sizeof(double));
         HIPC(hipMemcpy(iB_d_[d], iB[d].data(), K[d] * sizeof(double), hipMemcpyHostToDevice));
     }
-    alloc_((void **)&partials_, kRedMaxBlocks * 8 * sizeof(double));
-    alloc_((void **)&result_, 8 * sizeof(double));
-    alloc_((void **)&maxbits_, 64);
-    // FFT plans (one per axis; shared tables when lengths coincide are not
-    // worth the bookkeeping at these sizes)
     for (int d = 0; d < 3; ++d) {
         FftPlan &p = plan_[d];
         p.N = K[d];
@@ -154,7 +222,6 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
         std::vector<double2> twd(K[d]);
         std::vector<float2> twf(K[d]);
         for (int t = 0; t < K[d]; ++t) {
-            // exact reduction of the angle keeps FP32 twiddles accurate
             const double ang = -2.0 * M_PI * (double)t / (double)K[d];
             twd[t] = make_double2(cos(ang), sin(ang));
             twf[t] = make_float2((float)twd[t].x, (float)twd[t].y);
@@ -164,12 +231,85 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
         HIPC(hipMemcpy(p.tw_d, twd.data(), K[d] * sizeof(double2), hipMemcpyHostToDevice));
         HIPC(hipMemcpy(p.tw_f, twf.data(), K[d] * sizeof(float2), hipMemcpyHostToDevice));
     }
+
+    // descriptor: geometry, FFT plans, blocking
+    FrameDesc &D = desc_;
+    D.K0 = K[0];
+    D.K1 = K[1];
+    D.K2 = K[2];
+    D.ldim = ldim;
+    D.order = order;
+    D.grid = grid_;
+    D.iB0 = iB_d_[0];
+    D.iB1 = iB_d_[1];
+    D.iB2 = iB_d_[2];
+    D.tw0 = fp32 ? (const void *)plan_[0].tw_f : (const void *)plan_[0].tw_d;
+    D.tw1 = fp32 ? (const void *)plan_[1].tw_f : (const void *)plan_[1].tw_d;
+    D.tw2 = fp32 ? (const void *)plan_[2].tw_f : (const void *)plan_[2].tw_d;
+    D.plan0 = plan_args(plan_[0]);
+    D.plan1 = plan_args(plan_[1]);
+    D.plan2 = plan_args(plan_[2]);
+    D.mmax3 = mmax3;
+    D.fixed = fixed ? 1 : 0;
+    D.tabmode = kTableFull;
+    const int nrows = K[0] * K[1];
+    D.zC = fft_lines(fp32, K[2], (nrows + 1) / 2, D.plan2.generic, false);
+    {
+        int c = fft_lines(fp32, K[1], ldim, D.plan1.generic, true);
+        D.yT = (ldim + c - 1) / c;
+        D.yC = (ldim + D.yT - 1) / D.yT;
+        c = fft_lines(fp32, K[0], ldim, D.plan0.generic, true);
+        D.xT = (ldim + c - 1) / c;
+        D.xC = (ldim + D.xT - 1) / D.xT;
+    }
+    for (int d : {0, 1, 2}) {
+        const FftPlanArgs &pa = d == 0 ? D.plan0 : d == 1 ? D.plan1 : D.plan2;
+        const int C = d == 0 ? D.xC : d == 1 ? D.yC : D.zC;
+        const size_t maxel = fp32 ? fft_elems<float>() : fft_elems<double>();
+        if (lds_fft(fp32, K[d], C, pa.generic, d != 2) > kLdsBudget || (size_t)C * K[d] > maxel)
+            throw std::invalid_argument("grid dimension " + std::to_string(K[d]) +
+                                        " does not fit the FFT workgroup");
+    }
+    static const int tiles[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+    for (auto &t : tiles) {
+        const int tx = std::min(t[0], K[0]), ty = std::min(t[1], K[1]);
+        if ((size_t)tx * ty * K[2] * 8 <= kLdsBudget) {
+            D.tx = tx;
+            D.ty = ty;
+            break;
+        }
+    }
+    if (D.tx == 0) throw std::invalid_argument("K2 too large for the spreading tile");
+    D.ntx = (K[0] + D.tx - 1) / D.tx;
+    D.nty = (K[1] + D.ty - 1) / D.ty;
+
+    nred_infl_ = (int)std::min<size_t>((H() + kBlock - 1) / kBlock, 1024);
+    D.nred_i = nred_infl_;
+    const int nred_max = std::max(nred_infl_, K[1] * D.xT);
+    alloc_((void **)&d_desc_, sizeof(FrameDesc));
+    memset(&desc_uploaded_, 0xff, sizeof(FrameDesc));
+    alloc_((void **)&D.partials, (size_t)nred_max * 8 * sizeof(double));
+    alloc_((void **)&D.colcount, (size_t)nrows * sizeof(int));
+    alloc_((void **)&D.colstart, ((size_t)nrows + 1) * sizeof(int));
+    alloc_((void **)&D.maxbits, 64);
+    alloc_((void **)&D.fxscale, 64);
+    alloc_((void **)&result_, 8 * sizeof(double));
+    HIPC(hipMemsetAsync(D.colcount, 0, (size_t)nrows * sizeof(int), stream_));
+    HIPC(hipMemsetAsync(D.maxbits, 0, 64, stream_));
     HIPC(hipMemsetAsync(grid_, 0, grid_reals() * tsz, stream_));
+    HIPC(hipMemsetAsync(result_, 0, 8 * sizeof(double), stream_));
+    D.out = result_;
+    ensure_atoms(1024);
     HIPC(hipStreamSynchronize(stream_));
 }
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph_();
+    for (auto &r : prof_recs_) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
     for (void *p : allocs_) (void)hipFree(p);
     if (pin_) (void)hipHostFree(pin_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -188,6 +328,14 @@ void Engine::alloc_(void **p, size_t bytes) {
     dev_bytes_ += bytes;
 }
 
+void Engine::drop_graph_() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_n_ = -1;
+}
+
 void Engine::set_cell(const double L6_[6]) {
     for (int i = 0; i < 6; ++i) L6[i] = L6_[i];
     Cell c(L6);
@@ -198,6 +346,8 @@ void Engine::set_cell(const double L6_[6]) {
     cell.iL21 = c.iL[2][1];
     cell.iL22 = c.iL[2][2];
     cell.iV = c.iV;
+    desc_.cell = cell;
+    drop_graph_();
 }
 
 void Engine::upload_table(const double *A, const double *dA) {
@@ -206,9 +356,16 @@ void Engine::upload_table(const double *A, const double *dA) {
     HIPC(hipMemcpyAsync(table_, A, h * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIPC(hipMemcpyAsync(table_ + h, dA, 6 * h * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIPC(hipStreamSynchronize(stream_));
+    desc_.table = table_;
+    desc_.tabmode = kTableFull;
+    desc_.tcell = cell;
     has_table_ = true;
+    drop_graph_();
 }
 
+// Exact table by the reference's enumeration (every image in the ball),
+// then a check that the closed form evaluated from the indices reproduces
+// it; if so the kernels evaluate it on the fly and no table is read.
 void Engine::build_ewald_table(double eta, double max_m) {
     const size_t h = H();
     std::vector<double> tab(7 * h, 0.0);
@@ -232,24 +389,56 @@ void Engine::build_ewald_table(double eta, double max_m) {
                        d[4] += fp * mz * mx;
                        d[5] += fp * mz * my;
                    });
-    upload_table(A, dA);
+    double sumA = 0.0, sumD = 0.0, errA = 0.0, errD = 0.0;
+    for (int kx = 0; kx < K[0]; ++kx)
+        for (int ky = 0; ky < K[1]; ++ky)
+            for (int kz = 0; kz < ldim; ++kz) {
+                const size_t p = ((size_t)kx * K[1] + ky) * ldim + kz;
+                double a, d[6];
+                ewald_point_impl(K[0], K[1], K[2], cell, fac, max_m * max_m, kx, ky, kz, a, d);
+                sumA += fabs(A[p]);
+                errA = std::max(errA, fabs(a - A[p]));
+                double dm = 0.0;
+                for (int q = 0; q < 6; ++q) {
+                    dm = std::max(dm, fabs(dA[6 * p + q]));
+                    errD = std::max(errD, fabs(d[q] - dA[6 * p + q]));
+                }
+                sumD += dm;
+            }
+    const double tol = 1e-13;
+    const bool analytic = errA <= tol * sumA / (double)h && errD <= tol * sumD / (double)h;
+    if (analytic) {
+        desc_.tabmode = kTableAnalytic;
+        desc_.ewald_fac = fac;
+        desc_.max_m2 = max_m * max_m;
+        desc_.tcell = cell;
+        has_table_ = true;
+        drop_graph_();
+    } else {
+        upload_table(A, dA);
+    }
 }
 
-void Engine::ensure_atoms_(int n) {
+void Engine::ensure_atoms(int n) {
     if ((size_t)n <= atom_cap_) return;
-    size_t cap = std::max<size_t>(1024, (size_t)n + n / 4);
-    double *x, *w, *dx, *phi;
-    alloc_((void **)&x, 3 * cap * sizeof(double));
-    alloc_((void **)&w, cap * sizeof(double));
-    alloc_((void **)&dx, 3 * cap * sizeof(double));
-    alloc_((void **)&phi, cap * sizeof(double));
-    x_d_ = x;
-    w_d_ = w;
-    dx_d_ = dx;
-    phi_d_ = phi;
+    sync();
+    drop_graph_();
+    const size_t cap = std::max<size_t>(1024, (size_t)n + n / 4);
+    const size_t tsz = fp32 ? sizeof(float) : sizeof(double);
+    // staging for the host-pointer entry points and the sort workspace; the
+    // old buffers stay allocated until the engine is destroyed (rare growth)
+    alloc_((void **)&x_d_, 3 * cap * sizeof(double));
+    alloc_((void **)&w_d_, cap * sizeof(double));
+    alloc_((void **)&dx_d_, 3 * cap * sizeof(double));
+    alloc_((void **)&phi_d_, cap * sizeof(double));
+    alloc_((void **)&desc_.a_col, cap * sizeof(int));
+    alloc_((void **)&desc_.a_rank, cap * sizeof(int));
+    alloc_((void **)&desc_.s_frac, 3 * cap * tsz);
+    alloc_((void **)&desc_.s_w, cap * sizeof(double));
+    alloc_((void **)&desc_.s_cell, cap * sizeof(int));
+    alloc_((void **)&desc_.s_perm, cap * sizeof(int));
     atom_cap_ = cap;
-    // old staging buffers stay allocated until the engine dies (rare growth)
-    const size_t need = 4 * cap * sizeof(double);
+    const size_t need = 5 * cap * sizeof(double);
     if (need > pin_cap_) {
         if (pin_) HIPC(hipHostFree(pin_));
         HIPC(hipHostMalloc((void **)&pin_, need, hipHostMallocDefault));
@@ -257,229 +446,157 @@ void Engine::ensure_atoms_(int n) {
     }
 }
 
-// ---- spreading ---------------------------------------------------------------
-template <typename T>
-void Engine::spread_t_(int n, const double *w_d, const double *x_d) {
-    GridArgs g = grid_args(*this);
-    SplineArgs<T> sp = spline_args<T>(M, order);
-    const int blk = 256;
-    if (!fixed) {
-        ProfScope ps(this, "zero_grid");
-        zero_kernel<<<grid_blocks((long)grid_reals(), blk), blk, 0, stream_>>>(
-            (unsigned long long *)grid_, (long)grid_reals());
-        KCHECK("zero grid");
-    } else {
-        ProfScope ps(this, "spread_prep");
-        zero_kernel<<<grid_blocks((long)G(), blk), blk, 0, stream_>>>(acc_, (long)G());
-        zero_kernel<<<1, 64, 0, stream_>>>(maxbits_, 1);
-        absmax_bits<<<grid_blocks(n, blk, 1024), blk, 0, stream_>>>(n, w_d, maxbits_);
-        KCHECK("absmax");
-    }
-    if (n > 0) {
-        ProfScope ps(this, "spread");
-        const long work = (long)n * order;
-        const int nb = (int)((work + blk - 1) / blk);
-        double *gd = fixed ? nullptr : (double *)grid_;
-#define SPREAD_CASE(O)                                                                     \
-    case O:                                                                                \
-        if (fixed)                                                                         \
-            spread_kernel<T, O, 1><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, gd,  \
-                                                            acc_, maxbits_, mmax3);        \
-        else                                                                               \
-            spread_kernel<T, O, 0><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, gd,  \
-                                                            acc_, maxbits_, mmax3);        \
-        break;
-#define SPREAD_CASE_GENERIC                                                                \
-    if (fixed)                                                                             \
-        spread_kernel<T, 0, 1><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, gd, acc_, \
-                                                        maxbits_, mmax3);                  \
-    else                                                                                   \
-        spread_kernel<T, 0, 0><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, gd, acc_, \
-                                                        maxbits_, mmax3);                  \
-    break;
-        switch (order) {
-            SPREAD_CASE(4)
-            SPREAD_CASE(6)
-            SPREAD_CASE(8)
-            default: SPREAD_CASE_GENERIC
+void Engine::set_atoms_(int n, const double *w, const double *x, double *dx, double *phi,
+                        double *out) {
+    desc_.n = n;
+    desc_.w = w;
+    desc_.x = x;
+    desc_.dx = dx;
+    desc_.phi = phi;
+    desc_.out = out;
+}
+
+int Engine::stage_blocks(Stage s) const {
+    const FrameDesc &D = desc_;
+    const int nrows = K[0] * K[1];
+    switch (s) {
+        case ST_PREP:
+        case ST_SCATTER: return (D.n + kBlock - 1) / kBlock;
+        case ST_SCAN: return 1;
+        case ST_SPREAD: return D.ntx * D.nty;
+        case ST_ZR2C:
+        case ST_ZC2R: return (nrows + 2 * D.zC - 1) / (2 * D.zC);
+        case ST_YF:
+        case ST_YI: return K[0] * D.yT;
+        case ST_XF_SCALE:
+        case ST_XFUSED:
+        case ST_XI: return K[1] * D.xT;
+        case ST_INFL:
+        case ST_INFL_E: return nred_infl_;
+        case ST_FINISH: return 1;
+        case ST_GATHER:
+        case ST_GATHER_SORTED: {
+            const int apb = kBlock / order;
+            return (D.n + apb - 1) / apb;
         }
-#undef SPREAD_CASE
-#undef SPREAD_CASE_GENERIC
-        KCHECK("spread_kernel");
-    }
-    if (fixed) {
-        ProfScope ps(this, "fixed_to_grid");
-        fixed_to_grid<T><<<grid_blocks((long)G(), blk), blk, 0, stream_>>>(
-            acc_, (T *)grid_, g, n, maxbits_, mmax3);
-        KCHECK("fixed_to_grid");
+        default: return 0;
     }
 }
 
-// ---- FFTs ------------------------------------------------------------------------
-template <typename T>
-void Engine::fft_forward_t_() {
-    T *grid = (T *)grid_;
-    cplx<T> *cg = (cplx<T> *)grid_;
-    const int K0 = K[0], K1 = K[1], K2 = K[2];
-    {   // z: real rows -> half-complex
-        ProfScope ps(this, "fft_z_r2c");
-        const FftPlan &p = plan_[2];
-        const int C = lines_per_block<T>(K2, 16);
-        const int nrows = K0 * K1;
-        fft_z_r2c<T><<<(nrows + C - 1) / C, 256, lds_bytes<T>(K2, C), stream_>>>(
-            grid, nrows, ldim, plan_args(p), tw_of<T>(p), C);
-        KCHECK("fft_z_r2c");
-    }
-    {   // y: lines along y for each (x, kz)
-        ProfScope ps(this, "fft_y_fwd");
-        const FftPlan &p = plan_[1];
-        const int C = lines_per_block<T>(K1, sizeof(T) == 4 ? 32 : 16);
-        const int ntiles = (ldim + C - 1) / C;
-        fft_strided<T><<<K0 * ntiles, 256, lds_bytes<T>(K1, C), stream_>>>(
-            cg, ldim, ntiles, (long)K1 * ldim, (long)ldim, plan_args(p), tw_of<T>(p), C, -1,
-            nullptr, nullptr, nullptr);
-        KCHECK("fft_y_fwd");
-    }
-    {   // x: lines along x for each (ky, kz); folds 1/(b0 b1 b2) into the store
-        ProfScope ps(this, "fft_x_fwd");
-        const FftPlan &p = plan_[0];
-        const int C = lines_per_block<T>(K0, sizeof(T) == 4 ? 32 : 16);
-        const int ntiles = (ldim + C - 1) / C;
-        fft_strided<T><<<K1 * ntiles, 256, lds_bytes<T>(K0, C), stream_>>>(
-            cg, ldim, ntiles, (long)ldim, (long)K1 * ldim, plan_args(p), tw_of<T>(p), C, -1,
-            iB_d_[0], iB_d_[1], iB_d_[2]);
-        KCHECK("fft_x_fwd");
+size_t Engine::stage_lds(Stage s) const {
+    const FrameDesc &D = desc_;
+    switch (s) {
+        case ST_SPREAD: return (size_t)D.tx * D.ty * K[2] * 8;
+        case ST_ZR2C:
+        case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);
+        case ST_YF:
+        case ST_YI: return lds_fft(fp32, K[1], D.yC, D.plan1.generic, true);
+        case ST_XF_SCALE:
+        case ST_XFUSED:
+        case ST_XI: return lds_fft(fp32, K[0], D.xC, D.plan0.generic, true);
+        default: return 0;
     }
 }
 
-template <typename T>
-void Engine::fft_inverse_t_() {
-    T *grid = (T *)grid_;
-    cplx<T> *cg = (cplx<T> *)grid_;
-    const int K0 = K[0], K1 = K[1], K2 = K[2];
-    {
-        ProfScope ps(this, "fft_x_inv");
-        const FftPlan &p = plan_[0];
-        const int C = lines_per_block<T>(K0, sizeof(T) == 4 ? 32 : 16);
-        const int ntiles = (ldim + C - 1) / C;
-        fft_strided<T><<<K1 * ntiles, 256, lds_bytes<T>(K0, C), stream_>>>(
-            cg, ldim, ntiles, (long)ldim, (long)K1 * ldim, plan_args(p), tw_of<T>(p), C, +1,
-            nullptr, nullptr, nullptr);
-        KCHECK("fft_x_inv");
-    }
-    {
-        ProfScope ps(this, "fft_y_inv");
-        const FftPlan &p = plan_[1];
-        const int C = lines_per_block<T>(K1, sizeof(T) == 4 ? 32 : 16);
-        const int ntiles = (ldim + C - 1) / C;
-        fft_strided<T><<<K0 * ntiles, 256, lds_bytes<T>(K1, C), stream_>>>(
-            cg, ldim, ntiles, (long)K1 * ldim, (long)ldim, plan_args(p), tw_of<T>(p), C, +1,
-            nullptr, nullptr, nullptr);
-        KCHECK("fft_y_inv");
-    }
-    {
-        ProfScope ps(this, "fft_z_c2r");
-        const FftPlan &p = plan_[2];
-        const int C = lines_per_block<T>(K2, 16);
-        const int nrows = K0 * K1;
-        fft_z_c2r<T><<<(nrows + C - 1) / C, 256, lds_bytes<T>(K2, C), stream_>>>(
-            grid, nrows, ldim, plan_args(p), tw_of<T>(p), C);
-        KCHECK("fft_z_c2r");
+void Engine::launch(Stage s, const Launch &L, int blocks, size_t lds,
+                    hipStream_t st, int finish) {
+    if (blocks <= 0) return;
+    if (fp32) launch_order<float>(order, s, L, blocks, lds, st, sp_f_, fixed, finish);
+    else launch_order<double>(order, s, L, blocks, lds, st, sp_d_, fixed, finish);
+}
+
+void Engine::sync_desc_() {
+    if (memcmp(&desc_, &desc_uploaded_, sizeof(FrameDesc)) == 0) return;
+    // pageable source: the call returns once the bytes are staged
+    HIPC(hipMemcpyAsync(d_desc_, &desc_, sizeof(FrameDesc), hipMemcpyHostToDevice, stream_));
+    desc_uploaded_ = desc_;
+}
+
+void Engine::run_(Stage s, int finish) {
+    const int nb = stage_blocks(s);
+    if (nb <= 0) return;
+    sync_desc_();
+    ProfScope ps(this, stage_name(s));
+    launch(s, Launch{d_desc_, nullptr, 1}, nb, stage_lds(s), stream_, finish);
+}
+
+template <size_t NS>
+void Engine::run_seq_(const Stage (&seq)[NS]) {
+    Stage prev = ST_COUNT;
+    for (Stage s : seq) {
+        run_(s, finishes(prev, s));
+        prev = s;
     }
 }
 
-void Engine::fft_forward_() {
-    if (fp32) fft_forward_t_<float>();
-    else fft_forward_t_<double>();
-}
-void Engine::fft_inverse_() {
-    if (fp32) fft_inverse_t_<float>();
-    else fft_inverse_t_<double>();
-}
-
-// ---- influence + reductions -----------------------------------------------------
-template <typename T>
-void Engine::influence_t_(bool apply, double *out_d) {
-    GridArgs g = grid_args(*this);
-    const int nb = grid_blocks((long)H(), kRedBlock, kRedMaxBlocks);
-    cplx<T> *fq = (cplx<T> *)grid_;
-    {
-    ProfScope ps(this, apply ? "influence" : "energy");
-    if (apply)
-        influence_kernel<T, true><<<nb, kRedBlock, 0, stream_>>>(
-            fq, table_, table_ + H(), g, iB_d_[0], iB_d_[1], iB_d_[2], partials_);
-    else
-        influence_kernel<T, false><<<nb, kRedBlock, 0, stream_>>>(
-            fq, table_, table_ + H(), g, iB_d_[0], iB_d_[1], iB_d_[2], partials_);
-    KCHECK("influence_kernel");
-    }
-    ProfScope ps(this, "reduce_finalize");
-    reduce_finalize<<<1, kRedBlock, 0, stream_>>>(partials_, nb, cell.iV, out_d);
-    KCHECK("reduce_finalize");
-}
-
-// ---- gather ---------------------------------------------------------------------
-template <typename T>
-void Engine::gather_t_(int n, const double *w_d, const double *x_d, double *dx_d,
-                       double *phi_d) {
-    if (n <= 0) return;
-    GridArgs g = grid_args(*this);
-    SplineArgs<T> sp = spline_args<T>(M, order);
-    const int blk = 256;
-    const int nb = (n + blk - 1) / blk;
-    const T *grid = (const T *)grid_;
-    ProfScope ps(this, "gather");
-#define GATHER_CASE(O)                                                                    \
-    case O:                                                                               \
-        gather_kernel<T, O><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, grid, dx_d, \
-                                                     phi_d);                              \
-        break;
-    switch (order) {
-        GATHER_CASE(4)
-        GATHER_CASE(6)
-        GATHER_CASE(8)
-        default:
-            gather_kernel<T, 0><<<nb, blk, 0, stream_>>>(n, w_d, x_d, cell, g, sp, grid, dx_d,
-                                                         phi_d);
-    }
-#undef GATHER_CASE
-    KCHECK("gather_kernel");
-}
-
-// ---- public device-pointer operations ---------------------------------------------
+// ---- public device-pointer operations -----------------------------------------------
 void Engine::spread_forward(int n, const double *w_d, const double *x_d) {
-    if (fp32) spread_t_<float>(n, w_d, x_d);
-    else spread_t_<double>(n, w_d, x_d);
-    fft_forward_();
+    ensure_atoms(n);
+    set_atoms_(n, w_d, x_d, desc_.dx, desc_.phi, desc_.out);
+    run_seq_(kSfacStages);
     state = 0;
 }
 
 void Engine::energy(double *out_d) {
     if (state != 0) throw UsageError("You must call sfac before this routine.");
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
-    if (fp32) influence_t_<float>(false, out_d);
-    else influence_t_<double>(false, out_d);
+    desc_.out = out_d;
+    run_seq_(kEnStages);
 }
 
 void Engine::convolve(double *out_d) {
     if (state != 0) throw UsageError("You must call sfac before this routine.");
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
-    if (fp32) influence_t_<float>(true, out_d);
-    else influence_t_<double>(true, out_d);
-    fft_inverse_();
+    desc_.out = out_d;
+    run_seq_(kDe1Stages);
     state = 1;
 }
 
 void Engine::gather(int n, const double *w_d, const double *x_d, double *dx_d, double *phi_d) {
     if (state != 1) throw UsageError("You must call de1 before this routine.");
-    if (fp32) gather_t_<float>(n, w_d, x_d, dx_d, phi_d);
-    else gather_t_<double>(n, w_d, x_d, dx_d, phi_d);
+    ensure_atoms(n);
+    set_atoms_(n, w_d, x_d, dx_d, phi_d, desc_.out);
+    run_(ST_GATHER, 0);
 }
 
-// ---- host-pointer helpers -----------------------------------------------------------
+void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d,
+                      double *dx_d, double *phi_d) {
+    if (!has_table_) throw UsageError("No convolution kernel has been set.");
+    ensure_atoms(n);
+    set_atoms_(n, w_d, x_d, dx_d, phi_d, out_d);
+    const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.s_frac};
+    const bool graph = use_graph && !prof_;
+    if (graph) {
+        const bool same =
+            graph_exec_ && graph_n_ == n && memcmp(key, graph_key_, sizeof(key)) == 0;
+        if (!same) {
+            drop_graph_();
+            sync_desc_();  // the descriptor upload stays outside the graph
+            HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            try {
+                run_seq_(kEvalStages);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPC(hipStreamEndCapture(stream_, &graph_));
+            HIPC(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+            graph_n_ = n;
+            memcpy(graph_key_, key, sizeof(key));
+        }
+        HIPC(hipGraphLaunch(graph_exec_, stream_));
+    } else {
+        run_seq_(kEvalStages);
+    }
+    state = 1;
+}
+
+// ---- host-pointer helpers ----------------------------------------------------------------
 void Engine::sfac_host(int n, const double *w, const double *x) {
     if (n < 0) throw std::invalid_argument("n < 0");
-    ensure_atoms_(n);
+    ensure_atoms(n);
     // the caller's arrays are borrowed for the call only: copy them to
     // pinned staging, then upload asynchronously
     HIPC(hipStreamSynchronize(stream_));
@@ -512,7 +629,7 @@ double Engine::de1_host(double vir[6]) {
 void Engine::gather_host(int n, const double *w, const double *x, double *dx, double *phi) {
     if (state != 1) throw UsageError("You must call de1 before this routine.");
     if (n <= 0) return;
-    ensure_atoms_(n);
+    ensure_atoms(n);
     HIPC(hipStreamSynchronize(stream_));
     memcpy(pin_, x, 3 * (size_t)n * sizeof(double));
     if (w) memcpy(pin_ + 3 * (size_t)n, w, (size_t)n * sizeof(double));
@@ -529,6 +646,32 @@ void Engine::gather_host(int n, const double *w, const double *x, double *dx, do
         HIPC(hipMemcpyAsync(phi, phi_d_, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                             stream_));
     HIPC(hipStreamSynchronize(stream_));
+}
+
+double Engine::eval_host(int n, const double *w, const double *x, double vir[6], double *dx,
+                         double *phi) {
+    if (n < 0) throw std::invalid_argument("n < 0");
+    ensure_atoms(n);
+    HIPC(hipStreamSynchronize(stream_));
+    memcpy(pin_, x, 3 * (size_t)n * sizeof(double));
+    memcpy(pin_ + 3 * (size_t)n, w, (size_t)n * sizeof(double));
+    HIPC(hipMemcpyAsync(x_d_, pin_, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                        stream_));
+    HIPC(hipMemcpyAsync(w_d_, pin_ + 3 * (size_t)n, (size_t)n * sizeof(double),
+                        hipMemcpyHostToDevice, stream_));
+    evaluate(n, w_d_, x_d_, result_, dx ? dx_d_ : nullptr, phi ? phi_d_ : nullptr);
+    double out[8];
+    HIPC(hipMemcpyAsync(out, result_, 8 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (dx && n)
+        HIPC(hipMemcpyAsync(dx, dx_d_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    if (phi && n)
+        HIPC(hipMemcpyAsync(phi, phi_d_, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    if (vir)
+        for (int c = 0; c < 6; ++c) vir[c] = out[1 + c];
+    return out[0];
 }
 
 double *Engine::get_S_host() {
@@ -600,6 +743,107 @@ void Engine::profile_read(std::vector<std::string> &names, std::vector<double> &
         (void)hipEventDestroy(r.b);
     }
     prof_recs_.clear();
+}
+
+// ---- batches --------------------------------------------------------------------------------
+Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
+    if (frames_.empty()) throw std::invalid_argument("empty batch");
+    const Engine *e0 = frames_[0];
+    for (const Engine *e : frames_) {
+        if (e->fp32 != e0->fp32 || e->fixed != e0->fixed || e->order != e0->order)
+            throw std::invalid_argument(
+                "a batch needs one precision, accumulation mode and spline order");
+        if (!e->has_table()) throw UsageError("No convolution kernel has been set.");
+    }
+    const int nf = (int)frames_.size();
+    descs_.resize(nf);
+    for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
+    HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
+    HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
+}
+
+Batch::~Batch() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    (void)hipFree(d_descs_);
+    (void)hipFree(d_offs_);
+}
+
+void Batch::set_atoms(int f, int n, const double *w_d, const double *x_d, double *out_d,
+                      double *dx_d, double *phi_d) {
+    if (f < 0 || f >= (int)frames_.size()) throw std::invalid_argument("frame index");
+    Engine *e = frames_[f];
+    e->ensure_atoms(n);
+    FrameDesc &D = e->desc_mut();
+    D.n = n;
+    D.w = w_d;
+    D.x = x_d;
+    D.out = out_d;
+    D.dx = dx_d;
+    D.phi = phi_d;
+    descs_[f] = D;
+    dirty_ = true;
+}
+
+void Batch::upload_(hipStream_t st) {
+    const int nf = (int)frames_.size();
+    offs_.assign((size_t)ST_COUNT * (nf + 1), 0);
+    total_.assign(ST_COUNT, 0);
+    lds_.assign(ST_COUNT, 0);
+    for (int s = 0; s < ST_COUNT; ++s) {
+        int acc = 0;
+        for (int f = 0; f < nf; ++f) {
+            offs_[(size_t)s * (nf + 1) + f] = acc;
+            acc += frames_[f]->stage_blocks((Stage)s);
+            lds_[s] = std::max(lds_[s], frames_[f]->stage_lds((Stage)s));
+        }
+        offs_[(size_t)s * (nf + 1) + nf] = acc;
+        total_[s] = acc;
+    }
+    HIPC(hipMemcpyAsync(d_descs_, descs_.data(), nf * sizeof(FrameDesc), hipMemcpyHostToDevice,
+                        st));
+    HIPC(hipMemcpyAsync(d_offs_, offs_.data(), offs_.size() * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    dirty_ = false;
+}
+
+void Batch::evaluate(hipStream_t st) {
+    if (!st) st = frames_[0]->stream();  // the legacy null stream cannot be captured
+    if (dirty_) upload_(st);
+    const int nf = (int)frames_.size();
+    Engine *e0 = frames_[0];
+    auto run_all = [&]() {
+        Stage prev = ST_COUNT;
+        for (Stage s : kEvalStages) {
+            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf};
+            e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
+            prev = s;
+        }
+    };
+    if (use_graph) {
+        if (!graph_exec_) {
+            HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                run_all();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPC(hipStreamEndCapture(st, &graph_));
+            HIPC(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        }
+        HIPC(hipGraphLaunch(graph_exec_, st));
+    } else {
+        run_all();
+    }
+    for (Engine *e : frames_) e->state = 1;
 }
 
 }  // namespace spme

@@ -1,5 +1,6 @@
-// Device engine for one SPME system (one box, one grid, one spline order).
-// Host-only interface; the kernels live in engine.hip.
+// Device engine for one SPME system (one box, one grid, one spline order),
+// and batches of such systems launched together.  Host-only interface; the
+// kernels live in engine.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +17,23 @@ namespace spme {
 enum EngineFlags {
     kFlagFP32 = 1,        // spreading, FFTs and gather in FP32
     kFlagFixedPoint = 2,  // deterministic int64 fixed-point spreading (always on for FP32)
+};
+
+// Kernel stages.  An evaluation runs PREP..ZR2C, YF, XFUSED, YI, ZC2R,
+// GATHER_SORTED; the split entry points use XF_SCALE / INFL / INFL_E / XI.
+enum Stage {
+    ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
+    ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
+    ST_COUNT
+};
+const char *stage_name(Stage s);
+
+// Frames of one launch: frames[0..nf) in device memory, offs = per-frame
+// first workgroup (nf > 1 only).
+struct Launch {
+    const FrameDesc *frames;
+    const int *offs;
+    int nf;
 };
 
 struct FftPlan {
@@ -37,35 +55,46 @@ public:
     void upload_table(const double *A, const double *dA);  // H and 6H doubles
     void build_ewald_table(double eta, double max_m);      // closed-form table
     bool has_table() const { return has_table_; }
+    int table_mode() const { return desc_.tabmode; }
 
     // device-pointer operations (asynchronous on stream())
     void spread_forward(int n, const double *w_d, const double *x_d);  // -> state 0
     void energy(double *out_d);                                        // out[0]=E
     void convolve(double *out_d);  // out[0]=E, out[1..6]=virial; -> state 1
     void gather(int n, const double *w_d, const double *x_d, double *dx_d, double *phi_d);
+    // one evaluation with the fused reciprocal step; replayed from a captured
+    // hipGraph when called again with the same arguments and use_graph
+    void evaluate(int n, const double *w_d, const double *x_d, double *out_d, double *dx_d,
+                  double *phi_d);
 
     // host-pointer helpers (stage through pinned memory, synchronous results)
     void sfac_host(int n, const double *w, const double *x);
     double en_host();
     double de1_host(double vir[6]);
     void gather_host(int n, const double *w, const double *x, double *dx, double *phi);
+    double eval_host(int n, const double *w, const double *x, double vir[6], double *dx,
+                     double *phi);
     double *get_S_host();  // host mirror of the grid, K0 x K1 x ldim x 2 doubles
 
     void sync();
     hipStream_t stream() const { return stream_; }
     double *result_dev() const { return result_; }
+    bool use_graph = true;
 
     // Per-kernel timing with HIP events recorded on stream() around every
-    // launch while enabled.  profile_read synchronises and returns, per
-    // kernel name, the summed milliseconds and the launch count.
+    // launch while enabled (graph replay is off meanwhile).
     void profile_enable(bool on);
     void profile_read(std::vector<std::string> &names, std::vector<double> &ms,
                       std::vector<int> &count);
-    struct ProfScope {
-        Engine *e;
-        ProfScope(Engine *e_, const char *name) : e(e_) { e->prof_begin_(name); }
-        ~ProfScope() { e->prof_end_(); }
-    };
+
+    // descriptor and launch geometry (used by Batch)
+    const FrameDesc &desc() const { return desc_; }
+    FrameDesc &desc_mut() { return desc_; }
+    int stage_blocks(Stage s) const;
+    size_t stage_lds(Stage s) const;
+    void ensure_atoms(int n);
+    void launch(Stage s, const Launch &L, int blocks, size_t lds,
+                hipStream_t st, int finish = 0);
 
     int state = -1;  // -1 nothing, 0 S-hat, 1 real-space convolution
     int K[3];
@@ -84,35 +113,49 @@ public:
     size_t G() const { return (size_t)K[0] * K[1] * K[2]; }
     size_t device_bytes() const { return dev_bytes_; }
 
+    struct ProfScope {
+        Engine *e;
+        ProfScope(Engine *e_, const char *name) : e(e_) { e->prof_begin_(name); }
+        ~ProfScope() { e->prof_end_(); }
+    };
+
 private:
     void alloc_(void **p, size_t bytes);
-    void ensure_atoms_(int n);
-    void fft_forward_();
-    void fft_inverse_();
-    template <typename T> void fft_forward_t_();
-    template <typename T> void fft_inverse_t_();
-    template <typename T> void spread_t_(int n, const double *w_d, const double *x_d);
-    template <typename T> void gather_t_(int n, const double *w_d, const double *x_d,
-                                         double *dx_d, double *phi_d);
-    template <typename T> void influence_t_(bool apply, double *out_d);
+    void run_(Stage s, int finish);  // single-frame launch of one stage on stream_
+    template <size_t NS> void run_seq_(const Stage (&seq)[NS]);
+    void drop_graph_();
+    void set_atoms_(int n, const double *w, const double *x, double *dx, double *phi,
+                    double *out);
+
+    void sync_desc_();  // upload desc_ to d_desc_ when it changed
 
     hipStream_t stream_ = nullptr;
-    void *grid_ = nullptr;               // T[K0*K1*2*ldim]
-    unsigned long long *acc_ = nullptr;  // int64[K0*K1*K2] fixed-point accumulator
-    double *table_ = nullptr;            // A[H] then dA[6H]
+    FrameDesc desc_;
+    FrameDesc desc_uploaded_;
+    FrameDesc *d_desc_ = nullptr;
+    void *grid_ = nullptr;
+    double *table_ = nullptr;  // A[H] then dA[6H]
     bool has_table_ = false;
     double *iB_d_[3] = {nullptr, nullptr, nullptr};
-    double *partials_ = nullptr;
     double *result_ = nullptr;  // 8 doubles
-    unsigned long long *maxbits_ = nullptr;
+    int nred_infl_ = 1;
     double *x_d_ = nullptr, *w_d_ = nullptr, *dx_d_ = nullptr, *phi_d_ = nullptr;
     size_t atom_cap_ = 0;
+    size_t work_cap_ = 0;
     double *pin_ = nullptr;  // pinned staging
     size_t pin_cap_ = 0;
     std::vector<double> host_grid_;
     FftPlan plan_[3];
+    SplineArgs<float> sp_f_;
+    SplineArgs<double> sp_d_;
     size_t dev_bytes_ = 0;
     std::vector<void *> allocs_;
+
+    // captured evaluation
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    const void *graph_key_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int graph_n_ = -1;
 
     struct ProfRec {
         const char *name;
@@ -122,6 +165,35 @@ private:
     void prof_end_();
     bool prof_ = false;
     std::vector<ProfRec> prof_recs_;
+};
+
+// A set of engines with the same precision, accumulation mode and spline
+// order, evaluated together: one launch per stage covers every frame.
+class Batch {
+public:
+    explicit Batch(const std::vector<Engine *> &frames);
+    ~Batch();
+    // per-frame device pointers for the next evaluation
+    void set_atoms(int f, int n, const double *w_d, const double *x_d, double *out_d,
+                   double *dx_d, double *phi_d);
+    // all frames, one launch per stage; st == nullptr: frame 0's stream
+    void evaluate(hipStream_t st);
+    int size() const { return (int)frames_.size(); }
+    hipStream_t default_stream() const { return frames_[0]->stream(); }
+    bool use_graph = true;
+
+private:
+    void upload_(hipStream_t st);
+    std::vector<Engine *> frames_;
+    std::vector<FrameDesc> descs_;
+    FrameDesc *d_descs_ = nullptr;
+    int *d_offs_ = nullptr;                 // ST_COUNT x (nf + 1)
+    std::vector<int> offs_;                 // host copy
+    std::vector<int> total_;                // blocks per stage
+    std::vector<size_t> lds_;               // max LDS per stage
+    bool dirty_ = true;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
 };
 
 // thrown for API misuse; carries the reference's message

@@ -1,31 +1,36 @@
 // LDS-staged mixed-radix Stockham FFT line kernels (gfx950).
 //
-// One workgroup owns C lines of length N.  A line is staged once from HBM
-// into LDS (coalesced: along the line for the contiguous z axis, across
-// adjacent lines for the strided y and x axes), all radix stages run
-// LDS->LDS with ping-pong buffers (Stockham autosort, no bit reversal),
-// and the result is written back once.  Radix 2, 3, 4, 5 and 8 have
-// dedicated butterflies; any other prime factor uses a direct DFT whose
-// roots come from the same per-length twiddle table, so every N >= 1 is
-// supported (the reference accepts any K >= 1; its test uses K = 13).
+// A workgroup owns C lines of length N.  Lines are staged once from HBM into
+// one LDS buffer (coalesced: along the line for the contiguous z axis,
+// across adjacent kz lines for the strided y and x axes), every radix stage
+// runs in place with register staging (each thread reads its butterflies'
+// inputs, barrier, writes the outputs), twiddles come from an LDS copy of
+// the per-length table, and the result is written back once.  Staging
+// loads use clamped addresses with no branches, so every load of a thread
+// is in flight before the first one is waited for.
 //
-// Stage s with radix R and span Ns (product of earlier radices), for
-// j in [0, N/R):  k = j mod Ns;  v_r = in[j + r N/R] * W^(r k), W =
-// exp(sign 2 pi i/(Ns R));  v = DFT_R(v);  out[(j/Ns) Ns R + k + r Ns] = v_r.
+// Radix 2, 3, 4, 5 and 8 have dedicated butterflies; any other prime uses a
+// direct DFT (two LDS buffers for that stage), so every N >= 1 works (the
+// reference accepts any K >= 1; its own test uses K = 13).
 //
-// Twiddle table: tw[t] = exp(-2 pi i t/N) computed in double on the host
-// (rounded once for FP32); the inverse transform conjugates on the fly.
+// Stage with radix R and span Ns (product of earlier radices), j < N/R:
+//   k = j mod Ns;  v_r = in[j + r N/R] * W^(r k),  W = exp(sign 2 pi i/(Ns R));
+//   v = DFT_R(v);  out[(j/Ns) Ns R + k + r Ns] = v_r     (natural order out)
+//
+// Real transforms along z pack two real rows into one complex line
+// (z = a + i b) and separate the spectra afterwards, so the z passes do half
+// the complex work of a zero-imaginary transform.
 #pragma once
 
 #include "common.hpp"
 
 namespace spme {
 
-struct FftPlanArgs {
-    int N;
-    int nstages;
-    int radix[kMaxRadixStages];
-};
+// max complex elements a workgroup stages (bounds the per-thread registers
+// of a stage: 8 FP32 or 4 FP64 complex values per thread)
+template <typename T> __host__ __device__ constexpr int fft_elems() {
+    return sizeof(T) == 8 ? 1024 : 2048;
+}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
@@ -144,41 +149,58 @@ __device__ __forceinline__ void bfly8(cplx<T> *v, int sign) {
     v[7] = csub<T>(e[3], o3);
 }
 
-// One Stockham stage of fixed radix R over C lines held in LDS.
-// src/dst: line c occupies [c*LS, c*LS + N).
+// In-place stage of fixed radix R over C lines (line c at buf + c*LS):
+// every thread loads the inputs of up to MAXQ butterflies into registers,
+// transforms them, and after a barrier stores the outputs.
 template <typename T, int R>
-__device__ void stage_fixed(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst,
-                            int C, int LS, int N, int Ns, const cplx<T> *tw, int sign) {
+__device__ __forceinline__ void stage_ip(cplx<T> *buf, int C, int LS, int N, int Ns,
+                                         const cplx<T> *tw, int sign) {
+    constexpr int MAXQ = (fft_elems<T>() / kBlock + R - 1) / R;
     const int NR = N / R;
+    const int nb = C * NR;
     const int twstep = N / (Ns * R);
-    for (int b = threadIdx.x; b < C * NR; b += blockDim.x) {
-        const int c = b / NR;
-        const int j = b - c * NR;
-        const int k = j % Ns;
-        const cplx<T> *s = src + c * LS;
-        cplx<T> v[R];
+    cplx<T> v[MAXQ][R];
+    int dst[MAXQ];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = s[j + r * NR];
-        if (Ns > 1) {
+    for (int q = 0; q < MAXQ; ++q) {
+        const int bf = threadIdx.x + q * kBlock;
+        dst[q] = -1;
+        if (bf < nb) {
+            const int c = bf / NR;
+            const int j = bf - c * NR;
+            const int k = j % Ns;
+            const cplx<T> *s = buf + c * LS + j;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul<T>(v[r], twiddle<T>(tw, r * k * twstep, sign));
+            for (int r = 0; r < R; ++r) v[q][r] = s[r * NR];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[q][r] = cmul<T>(v[q][r], twiddle<T>(tw, r * k * twstep, sign));
+            }
+            if (R == 2) bfly2<T>(v[q]);
+            else if (R == 3) bfly3<T>(v[q], sign);
+            else if (R == 4) bfly4<T>(v[q], sign);
+            else if (R == 5) bfly5<T>(v[q], sign);
+            else if (R == 8) bfly8<T>(v[q], sign);
+            dst[q] = c * LS + (j / Ns) * Ns * R + k;
         }
-        if (R == 2) bfly2<T>(v);
-        else if (R == 3) bfly3<T>(v, sign);
-        else if (R == 4) bfly4<T>(v, sign);
-        else if (R == 5) bfly5<T>(v, sign);
-        else if (R == 8) bfly8<T>(v, sign);
-        cplx<T> *d = dst + c * LS + (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
 }
 
-// Generic radix (any R dividing N): direct DFT, roots from the N-table.
+// Generic radix (any R dividing N): direct DFT from src to dst.
 template <typename T>
-__device__ void stage_generic(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst,
-                              int C, int LS, int N, int Ns, int R, const cplx<T> *tw,
-                              int sign) {
+__device__ __forceinline__ void stage_generic(const cplx<T> *__restrict__ src,
+                                              cplx<T> *__restrict__ dst, int C, int LS, int N,
+                                              int Ns, int R, const cplx<T> *tw, int sign) {
     const int NR = N / R;
     const int twstep = N / (Ns * R);
     for (int b = threadIdx.x; b < C * NR; b += blockDim.x) {
@@ -200,138 +222,202 @@ __device__ void stage_generic(const cplx<T> *__restrict__ src, cplx<T> *__restri
             d[q * Ns] = acc;
         }
     }
+    __syncthreads();
 }
 
-// Runs all stages; returns the LDS buffer holding the result.
+// All stages of the plan; returns the buffer that holds the result.
 template <typename T>
-__device__ cplx<T> *run_stages(cplx<T> *b0, cplx<T> *b1, int C, int LS,
-                               const FftPlanArgs &p, const cplx<T> *tw, int sign) {
+__device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, int LS,
+                                              const FftPlanArgs &p, const cplx<T> *tw,
+                                              int sign) {
     int Ns = 1;
-    cplx<T> *src = b0, *dst = b1;
+    cplx<T> *cur = b0, *oth = b1;
     for (int s = 0; s < p.nstages; ++s) {
         const int R = p.radix[s];
         switch (R) {
-            case 2: stage_fixed<T, 2>(src, dst, C, LS, p.N, Ns, tw, sign); break;
-            case 3: stage_fixed<T, 3>(src, dst, C, LS, p.N, Ns, tw, sign); break;
-            case 4: stage_fixed<T, 4>(src, dst, C, LS, p.N, Ns, tw, sign); break;
-            case 5: stage_fixed<T, 5>(src, dst, C, LS, p.N, Ns, tw, sign); break;
-            case 8: stage_fixed<T, 8>(src, dst, C, LS, p.N, Ns, tw, sign); break;
-            default: stage_generic<T>(src, dst, C, LS, p.N, Ns, R, tw, sign); break;
+            case 2: stage_ip<T, 2>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 3: stage_ip<T, 3>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 4: stage_ip<T, 4>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 5: stage_ip<T, 5>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 8: stage_ip<T, 8>(cur, C, LS, p.N, Ns, tw, sign); break;
+            default: {
+                stage_generic<T>(cur, oth, C, LS, p.N, Ns, R, tw, sign);
+                cplx<T> *t = cur;
+                cur = oth;
+                oth = t;
+            }
         }
-        __syncthreads();
         Ns *= R;
-        cplx<T> *t = src;
-        src = dst;
-        dst = t;
     }
-    return src;
+    return cur;
+}
+
+// Twiddles to LDS: all loads first (clamped index), then the LDS writes.
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cplx<T> *twl, const void *tw, int N) {
+    const cplx<T> *g = (const cplx<T> *)tw;
+    constexpr int Q = (kMaxK + kBlock - 1) / kBlock;
+    T vx[Q], vy[Q];  // scalar arrays stay in registers (vector-typed ones may not)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const cplx<T> w = g[min((int)threadIdx.x + q * kBlock, N - 1)];
+        vx[q] = w.x;
+        vy[q] = w.y;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int t = threadIdx.x + q * kBlock;
+        if (t < N) {
+            cplx<T> w;
+            w.x = vx[q];
+            w.y = vy[q];
+            twl[t] = w;
+        }
+    }
+}
+
+// LDS bytes of a line workgroup: twiddles + C lines (+ a second buffer for
+// plans with a generic radix)
+template <typename T>
+__host__ __device__ inline size_t fft_lds_bytes(int N, int C, int generic) {
+    return ((size_t)N + (size_t)C * (N + 1) * (generic ? 2 : 1)) * sizeof(cplx<T>);
 }
 
 // ---------------------------------------------------------------------------
-// z axis, real -> half-complex.  rows = K0*K1, each of K2 reals at stride
-// 2*ldim reals; output ldim complex in place.  Block: C rows.
+// z axis, real -> half-complex, two real rows per complex line.
+// Rows r = 0..K0*K1-1, K2 reals each at stride 2*ldim; the spectra of rows
+// 2c and 2c+1 come out of Z = FFT(a + i b) as
+//   A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i).
 template <typename T>
-__global__ void __launch_bounds__(256)
-fft_z_r2c(T *__restrict__ grid, int nrows, int ldim, FftPlanArgs p,
-          const cplx<T> *__restrict__ tw, int C) {
+__global__ void __launch_bounds__(kBlock) fft_z_r2c_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cplx<T> *b0 = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int N = p.N;
-    const int LS = N + 1;
-    cplx<T> *b1 = b0 + C * LS;
-    const int row0 = blockIdx.x * C;
-    const int nr = min(C, nrows - row0);
-    const long rs = 2L * ldim;  // row stride in reals
-    for (int l = threadIdx.x; l < nr * N; l += blockDim.x) {
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const FftPlanArgs &p = F.plan2;
+    const int N = p.N, ldim = F.ldim, LS = N + 1;
+    const int nrows = F.K0 * F.K1;
+    const int zC = F.zC;
+    const int row0 = b * 2 * zC;
+    const int nl = min(zC, (nrows - row0 + 1) / 2);
+    cplx<T> *twl = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *b0 = twl + N;
+    cplx<T> *b1 = b0 + zC * LS;
+    load_twiddles<T>(twl, F.tw2, N);
+    const T *g = (const T *)F.grid;
+    const long rs = 2L * ldim;
+    constexpr int EPT = fft_elems<T>() / kBlock;
+    const int tot = nl * N;
+    T ra[EPT], rb[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int l = min((int)threadIdx.x + q * kBlock, tot - 1);
         const int c = l / N, z = l - c * N;
-        cplx<T> v;
-        v.x = grid[(long)(row0 + c) * rs + z];
-        v.y = T(0);
-        b0[c * LS + z] = v;
+        const int r = row0 + 2 * c;
+        ra[q] = g[(long)r * rs + z];
+        rb[q] = g[(long)min(r + 1, nrows - 1) * rs + z];
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int l = threadIdx.x + q * kBlock;
+        if (l < tot) {
+            const int c = l / N, z = l - c * N;
+            cplx<T> v;
+            v.x = ra[q];
+            v.y = (row0 + 2 * c + 1 < nrows) ? rb[q] : T(0);
+            b0[c * LS + z] = v;
+        }
     }
     __syncthreads();
-    cplx<T> *res = run_stages<T>(b0, b1, nr, LS, p, tw, -1);
-    cplx<T> *out = reinterpret_cast<cplx<T> *>(grid);
-    for (int l = threadIdx.x; l < nr * ldim; l += blockDim.x) {
+    const cplx<T> *res = fft_lines<T>(b0, b1, nl, LS, p, twl, -1);
+    cplx<T> *out = (cplx<T> *)F.grid;
+    for (int l = threadIdx.x; l < nl * ldim; l += blockDim.x) {
         const int c = l / ldim, k = l - c * ldim;
-        out[(long)(row0 + c) * ldim + k] = res[c * LS + k];
+        const cplx<T> zk = res[c * LS + k];
+        const cplx<T> zm = res[c * LS + (k == 0 ? 0 : N - k)];
+        const int r = row0 + 2 * c;
+        cplx<T> a, bb;
+        a.x = T(0.5) * (zk.x + zm.x);
+        a.y = T(0.5) * (zk.y - zm.y);
+        bb.x = T(0.5) * (zk.y + zm.y);
+        bb.y = T(0.5) * (zm.x - zk.x);
+        out[(long)r * ldim + k] = a;
+        if (r + 1 < nrows) out[(long)(r + 1) * ldim + k] = bb;
     }
 }
 
-// z axis, half-complex -> real (unnormalised, sign +1).  Hermitian
-// completion X[K2-k] = conj(X[k]); the real part of the full complex
-// transform equals the c2r result (imaginary parts of the kz=0 and Nyquist
-// terms contribute sin(0) = sin(pi z) = 0).
+// z axis, half-complex -> real (unnormalised, sign +1), two rows per line:
+// Z_k = A_k + i B_k over the Hermitian-completed rows, real part -> row 2c,
+// imaginary part -> row 2c+1.  Imaginary parts of the kz = 0 and Nyquist
+// terms are dropped, as a c2r transform ignores them.
 template <typename T>
-__global__ void __launch_bounds__(256)
-fft_z_c2r(T *__restrict__ grid, int nrows, int ldim, FftPlanArgs p,
-          const cplx<T> *__restrict__ tw, int C) {
+__global__ void __launch_bounds__(kBlock) fft_z_c2r_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cplx<T> *b0 = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int N = p.N;
-    const int LS = N + 1;
-    cplx<T> *b1 = b0 + C * LS;
-    const int row0 = blockIdx.x * C;
-    const int nr = min(C, nrows - row0);
-    const cplx<T> *in = reinterpret_cast<const cplx<T> *>(grid);
-    for (int l = threadIdx.x; l < nr * ldim; l += blockDim.x) {
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const FftPlanArgs &p = F.plan2;
+    const int N = p.N, ldim = F.ldim, LS = N + 1;
+    const int nrows = F.K0 * F.K1;
+    const int zC = F.zC;
+    const int row0 = b * 2 * zC;
+    const int nl = min(zC, (nrows - row0 + 1) / 2);
+    cplx<T> *twl = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *b0 = twl + N;
+    cplx<T> *b1 = b0 + zC * LS;
+    load_twiddles<T>(twl, F.tw2, N);
+    const cplx<T> *in = (const cplx<T> *)F.grid;
+    const bool even = (N % 2) == 0;
+    constexpr int EPT = fft_elems<T>() / kBlock;
+    const int tot = nl * ldim;
+    T rax[EPT], ray[EPT], rbx[EPT], rby[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int l = min((int)threadIdx.x + q * kBlock, tot - 1);
         const int c = l / ldim, k = l - c * ldim;
-        const cplx<T> v = in[(long)(row0 + c) * ldim + k];
-        b0[c * LS + k] = v;
+        const int r = row0 + 2 * c;
+        const cplx<T> va = in[(long)r * ldim + k];
+        const cplx<T> vb = in[(long)min(r + 1, nrows - 1) * ldim + k];
+        rax[q] = va.x;
+        ray[q] = va.y;
+        rbx[q] = vb.x;
+        rby[q] = vb.y;
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int l = threadIdx.x + q * kBlock;
+        if (l >= tot) continue;
+        const int c = l / ldim, k = l - c * ldim;
+        cplx<T> a, bb;
+        a.x = rax[q];
+        a.y = ray[q];
+        bb.x = rbx[q];
+        bb.y = rby[q];
+        if (row0 + 2 * c + 1 >= nrows) bb.x = bb.y = T(0);
+        if (k == 0 || (even && k == N / 2)) {
+            a.y = T(0);
+            bb.y = T(0);
+        }
+        cplx<T> z;
+        z.x = a.x - bb.y;
+        z.y = a.y + bb.x;
+        b0[c * LS + k] = z;
         const int km = N - k;
         if (k > 0 && km >= ldim) {
-            cplx<T> w;
-            w.x = v.x;
-            w.y = -v.y;
-            b0[c * LS + km] = w;
+            cplx<T> zc;  // conj(A) + i conj(B)
+            zc.x = a.x + bb.y;
+            zc.y = bb.x - a.y;
+            b0[c * LS + km] = zc;
         }
     }
     __syncthreads();
-    cplx<T> *res = run_stages<T>(b0, b1, nr, LS, p, tw, +1);
+    const cplx<T> *res = fft_lines<T>(b0, b1, nl, LS, p, twl, +1);
+    T *g = (T *)F.grid;
     const long rs = 2L * ldim;
-    for (int l = threadIdx.x; l < nr * N; l += blockDim.x) {
+    for (int l = threadIdx.x; l < nl * N; l += blockDim.x) {
         const int c = l / N, z = l - c * N;
-        grid[(long)(row0 + c) * rs + z] = res[c * LS + z].x;
-    }
-}
-
-// Strided axis (y or x) complex transform over the half-complex array.
-// Line (o, c): element e at o*ostride + c + e*estride (complex units),
-// c in [0, ninner).  Block = (o, tile of C adjacent c), so each element
-// row of the tile is a contiguous run of C complex numbers in HBM.
-// Optional per-output scale sc_e[e] * sc_o[o] * sc_c[c] (used to fold the
-// B-spline moduli 1/(b0 b1 b2) into the last forward pass).
-template <typename T>
-__global__ void __launch_bounds__(256)
-fft_strided(cplx<T> *__restrict__ data, int ninner, int ntiles, long ostride,
-            long estride, FftPlanArgs p, const cplx<T> *__restrict__ tw, int C,
-            int sign, const double *__restrict__ sc_e, const double *__restrict__ sc_o,
-            const double *__restrict__ sc_c) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cplx<T> *b0 = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int N = p.N;
-    const int LS = N + 1;
-    cplx<T> *b1 = b0 + C * LS;
-    const int o = blockIdx.x / ntiles;
-    const int c0 = (blockIdx.x - o * ntiles) * C;
-    const int nc = min(C, ninner - c0);
-    cplx<T> *base = data + (long)o * ostride + c0;
-    for (int l = threadIdx.x; l < nc * N; l += blockDim.x) {
-        const int e = l / nc, c = l - e * nc;
-        b0[c * LS + e] = base[(long)e * estride + c];
-    }
-    __syncthreads();
-    cplx<T> *res = run_stages<T>(b0, b1, nc, LS, p, tw, sign);
-    const double so = sc_o ? sc_o[o] : 1.0;
-    for (int l = threadIdx.x; l < nc * N; l += blockDim.x) {
-        const int e = l / nc, c = l - e * nc;
-        cplx<T> v = res[c * LS + e];
-        if (sc_e) {
-            const T s = T(sc_e[e] * so * sc_c[c0 + c]);
-            v.x *= s;
-            v.y *= s;
-        }
-        base[(long)e * estride + c] = v;
+        const int r = row0 + 2 * c;
+        const cplx<T> v = res[c * LS + z];
+        g[(long)r * rs + z] = v.x;
+        if (r + 1 < nrows) g[(long)(r + 1) * rs + z] = v.y;
     }
 }
 

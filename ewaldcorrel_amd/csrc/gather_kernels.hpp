@@ -1,0 +1,122 @@
+// Force and potential gather (de2 / potl, sfac.cpp:409-545).
+//
+// Per atom: u = scale(x) K, k1 = ceil(u), t = k1 - u, stencil grid points
+// (k1 - n/2 + i) mod K with weights M(t+i) and derivatives M'(t+i);
+//   dE/ds_d = -w iV K_d sum d_d(prod M) Q,  dx = iL (dE/ds)   (force = -dx)
+//   phi     = iV sum prod M Q                                  (= dE/dw)
+// n threads per atom, thread i owns the x-plane i of the stencil (n^2 loads,
+// n consecutive reals per row); the planes are summed through LDS.  With
+// SORTED the workgroups walk the atoms in the column order of the last
+// spreading pass, so neighbouring atoms share grid lines in the caches.
+#pragma once
+
+#include "common.hpp"
+#include "spread_kernels.hpp"
+
+namespace spme {
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void spline_wd(const SplineArgs<T> &sp, int n, T x, T *b, T *db) {
+#pragma unroll
+    for (int j = 0; j < Ord<ORDER>::cap; ++j) {
+        if (j >= n) break;
+        T d = sp.M[j * n];
+        T v = sp.M[j * n] * x + sp.M[j * n + 1];
+#pragma unroll
+        for (int i = 2; i < Ord<ORDER>::cap; ++i) {
+            if (i >= n) break;
+            d = d * x + v;
+            v = v * x + sp.M[j * n + i];
+        }
+        b[j] = v;
+        db[j] = d;
+    }
+}
+
+template <typename T, int ORDER, bool SORTED>
+__global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
+    __shared__ double red[kBlock][4];
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const int n = order_of<ORDER>(F.order);
+    constexpr int CAP = Ord<ORDER>::cap;
+    const int apb = kBlock / n;
+    const int slot = threadIdx.x / n;
+    const int i = threadIdx.x - slot * n;
+    const int p = b * apb + slot;
+    const bool valid = slot < apb && p < F.n;
+    double d0 = 0, d1 = 0, d2 = 0, pp = 0;
+    int a = 0;
+    if (valid) {
+        a = SORTED ? F.s_perm[p] : p;
+        double s0, s1, s2;
+        scale_wrap(F.cell, F.x[3 * a], F.x[3 * a + 1], F.x[3 * a + 2], s0, s1, s2);
+        const double u0 = s0 * F.K0, u1 = s1 * F.K1, u2 = s2 * F.K2;
+        const double k0 = ceil(u0), k1 = ceil(u1), k2 = ceil(u2);
+        T by[CAP], bz[CAP], dby[CAP], dbz[CAP], bx[CAP], dbx[CAP];
+        spline_wd<T, ORDER>(sp, n, T(k0 - u0), bx, dbx);
+        spline_wd<T, ORDER>(sp, n, T(k1 - u1), by, dby);
+        spline_wd<T, ORDER>(sp, n, T(k2 - u2), bz, dbz);
+        const int gx = imod((int)k0 - n / 2 + i, F.K0);
+        const int o1 = imod((int)k1 - n / 2, F.K1);
+        const int o2 = imod((int)k2 - n / 2, F.K2);
+        const T *grid = (const T *)F.grid;
+        const long rs = 2L * F.ldim;
+        T s_b = 0, s_db = 0, s_bdz = 0;
+#pragma unroll
+        for (int j = 0; j < CAP; ++j) {
+            if (j >= n) break;
+            int gy = o1 + j;
+            gy = gy >= F.K1 ? gy - F.K1 * (gy / F.K1) : gy;
+            const T *row = grid + ((long)gx * F.K1 + gy) * rs;
+            T zb = 0, zdb = 0;
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) {
+                if (k >= n) break;
+                int z = o2 + k;
+                z = z >= F.K2 ? z - F.K2 * (z / F.K2) : z;
+                const T q = row[z];
+                zb += bz[k] * q;
+                zdb += dbz[k] * q;
+            }
+            s_b += by[j] * zb;
+            s_db += dby[j] * zb;
+            s_bdz += by[j] * zdb;
+        }
+        T bxi = bx[0], dbxi = dbx[0];
+#pragma unroll
+        for (int q = 1; q < CAP; ++q)
+            if (q == i) {
+                bxi = bx[q];
+                dbxi = dbx[q];
+            }
+        d0 = (double)(dbxi * s_b);
+        d1 = (double)(bxi * s_db);
+        d2 = (double)(bxi * s_bdz);
+        pp = (double)(bxi * s_b);
+    }
+    red[threadIdx.x][0] = d0;
+    red[threadIdx.x][1] = d1;
+    red[threadIdx.x][2] = d2;
+    red[threadIdx.x][3] = pp;
+    __syncthreads();
+    if (!valid || i != 0) return;
+    double e0 = 0, e1 = 0, e2 = 0, ph = 0;
+    for (int q = 0; q < n; ++q) {
+        e0 += red[threadIdx.x + q][0];
+        e1 += red[threadIdx.x + q][1];
+        e2 += red[threadIdx.x + q][2];
+        ph += red[threadIdx.x + q][3];
+    }
+    const CellArgs &c = F.cell;
+    if (F.dx) {
+        const double f = -F.w[a] * c.iV;
+        const double g0 = e0 * f * F.K0, g1 = e1 * f * F.K1, g2 = e2 * f * F.K2;
+        F.dx[3 * a + 0] = c.iL00 * g0;
+        F.dx[3 * a + 1] = c.iL10 * g0 + c.iL11 * g1;
+        F.dx[3 * a + 2] = c.iL20 * g0 + c.iL21 * g1 + c.iL22 * g2;
+    }
+    if (F.phi) F.phi[a] = ph * c.iV;
+}
+
+}  // namespace spme

@@ -244,10 +244,7 @@ int sfac_eval_dev(void *h, int n, const double *w_d, const double *x_d, double *
         if (n < 0) throw std::invalid_argument("n < 0");
         SFac *s = as_sfac(h);
         spme::Engine *e = s->engine;
-        if (!e->has_table()) throw spme::UsageError("No convolution kernel has been set.");
-        e->spread_forward(n, w_d, x_d);
-        e->convolve(out_d ? out_d : e->result_dev());
-        e->gather(n, w_d, x_d, dx_d, phi_d);
+        e->evaluate(n, w_d, x_d, out_d ? out_d : e->result_dev(), dx_d, phi_d);
         s->state = e->state;
     });
 }
@@ -257,15 +254,14 @@ double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6]
     double E = 0.0;
     guard([&] {
         SFac *s = as_sfac(h);
-        s->engine->sfac_host(n, w, x);
-        double v[6];
-        E = s->engine->de1_host(v);
-        if (vir)
-            for (int c = 0; c < 6; ++c) vir[c] = v[c];
-        s->engine->gather_host(n, w, x, dx, phi);
+        E = s->engine->eval_host(n, w, x, vir, dx, phi);
         s->state = s->engine->state;
     });
     return E;
+}
+
+int sfac_set_graph(void *h, int enable) {
+    return guard([&] { as_sfac(h)->engine->use_graph = enable != 0; });
 }
 
 void *sfac_stream(void *h) {
@@ -278,7 +274,7 @@ int sfac_sync(void *h) {
     return guard([&] { as_sfac(h)->engine->sync(); });
 }
 
-int sfac_info(void *h, long long out[8]) {
+int sfac_info(void *h, long long out[10]) {
     return guard([&] {
         spme::Engine *e = as_sfac(h)->engine;
         out[0] = e->K[0];
@@ -289,6 +285,8 @@ int sfac_info(void *h, long long out[8]) {
         out[5] = (e->fp32 ? SFAC_FP32 : 0) | (e->fixed ? SFAC_FIXED : 0);
         out[6] = (long long)e->device_bytes();
         out[7] = e->state;
+        out[8] = e->table_mode();
+        out[9] = e->use_graph ? 1 : 0;
     });
 }
 
@@ -317,6 +315,55 @@ int sfac_profile_read(void *h, char *names, int names_len, double *ms, int *coun
         }
     });
     return rc == 0 ? n : rc;
+}
+
+// ---- batches ------------------------------------------------------------------------------
+void *sfac_batch_create(void **handles, int nframes) {
+    spme::Batch *b = nullptr;
+    int rc = guard([&] {
+        if (nframes < 1) throw std::invalid_argument("nframes < 1");
+        std::vector<spme::Engine *> fr;
+        for (int f = 0; f < nframes; ++f) fr.push_back(as_sfac(handles[f])->engine);
+        b = new spme::Batch(fr);
+    });
+    return rc == 0 ? reinterpret_cast<void *>(b) : nullptr;
+}
+
+int sfac_batch_set(void *b, int f, int n, const double *w_d, const double *x_d, double *out_d,
+                   double *dx_d, double *phi_d) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        if (n < 0) throw std::invalid_argument("n < 0");
+        if (!out_d) throw std::invalid_argument("out_d is required");
+        reinterpret_cast<spme::Batch *>(b)->set_atoms(f, n, w_d, x_d, out_d, dx_d, phi_d);
+    });
+}
+
+int sfac_batch_eval(void *b, void *stream) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        reinterpret_cast<spme::Batch *>(b)->evaluate(reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int sfac_batch_sync(void *b) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        spme::hip_check(
+            hipStreamSynchronize(reinterpret_cast<spme::Batch *>(b)->default_stream()),
+            "hipStreamSynchronize");
+    });
+}
+
+int sfac_batch_set_graph(void *b, int enable) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        reinterpret_cast<spme::Batch *>(b)->use_graph = enable != 0;
+    });
+}
+
+int sfac_batch_destroy(void *b) {
+    return guard([&] { delete reinterpret_cast<spme::Batch *>(b); });
 }
 
 }  // extern "C"

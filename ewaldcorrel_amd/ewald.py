@@ -35,6 +35,8 @@ SFAC_FP32 = 1
 SFAC_FIXED = 2
 
 _LIB = None
+# double (*rad_fn)(double r2, double *dA, void *info), called with the GIL held
+_RAD_FN = PYFUNCTYPE(c_double, c_double, POINTER(c_double), c_void_p)
 
 
 def _libpath():
@@ -70,7 +72,7 @@ def describe_sfac():
         return np.ctypeslib.ndpointer(dtype=np.int32, shape=shape, flags='C_CONTIGUOUS')
 
     sfac_p = c_void_p
-    rad_fn = PYFUNCTYPE(c_double, c_double, POINTER(c_double), c_void_p)
+    rad_fn = _RAD_FN  # one type for the cached library's argtypes
 
     if _LIB is None:
         path = _libpath()
@@ -104,6 +106,14 @@ def describe_sfac():
         decl_fn(lib.sfac_stream, sfac_p, c_void_p)
         int_fn(lib.sfac_sync, sfac_p)
         int_fn(lib.sfac_info, sfac_p, POINTER(c_longlong))
+        int_fn(lib.sfac_set_graph, sfac_p, c_int)
+        decl_fn(lib.sfac_batch_create, POINTER(c_void_p), c_int, c_void_p)
+        int_fn(lib.sfac_batch_set, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+               c_void_p, c_void_p)
+        int_fn(lib.sfac_batch_eval, c_void_p, c_void_p)
+        int_fn(lib.sfac_batch_sync, c_void_p)
+        int_fn(lib.sfac_batch_set_graph, c_void_p, c_int)
+        int_fn(lib.sfac_batch_destroy, c_void_p)
         int_fn(lib.sfac_profile, sfac_p, c_int)
         int_fn(lib.sfac_profile_read, sfac_p, c_char_p, c_int, POINTER(c_double),
                POINTER(c_int), c_int)
@@ -250,11 +260,55 @@ class Sfac:
         keys = names.value.decode().split("\n") if n else []
         return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
 
+    def set_graph(self, enable=True):
+        """Replay eval_dev/evaluate from a captured hipGraph (default on)."""
+        _check(self._libsfac.sfac_set_graph(self._data, int(bool(enable))), "set_graph")
+
     def info(self):
-        out = (c_longlong * 8)()
+        out = (c_longlong * 10)()
         _check(self._libsfac.sfac_info(self._data, out), "info")
-        keys = ("K0", "K1", "K2", "ldim", "order", "flags", "device_bytes", "state")
+        keys = ("K0", "K1", "K2", "ldim", "order", "flags", "device_bytes", "state",
+                "table_mode", "graph")
         return dict(zip(keys, list(out)))
+
+
+class SfacBatch:
+    """Frames (Sfac handles with one precision, accumulation mode and spline
+    order, influence functions set) evaluated together: one launch per stage
+    for all frames, replayed from a captured hipGraph while the per-frame
+    device pointers stay the same."""
+
+    def __init__(self, frames):
+        _, _, lib = describe_sfac()
+        self._lib = lib
+        self.frames = list(frames)
+        arr = (c_void_p * len(self.frames))(*[f._data for f in self.frames])
+        self._b = lib.sfac_batch_create(arr, len(self.frames))
+        if not self._b:
+            raise RuntimeError("sfac_batch_create failed: " + last_error())
+
+    def set(self, f, n, w_ptr, x_ptr, out_ptr, dx_ptr=None, phi_ptr=None):
+        _check(self._lib.sfac_batch_set(self._b, f, n, w_ptr, x_ptr, out_ptr, dx_ptr,
+                                        phi_ptr), "sfac_batch_set")
+
+    def eval(self, stream=None):
+        """One evaluation of every frame on `stream` (None: the first
+        frame's stream, waited for by sync())."""
+        _check(self._lib.sfac_batch_eval(self._b, stream), "sfac_batch_eval")
+
+    def sync(self):
+        _check(self._lib.sfac_batch_sync(self._b), "sfac_batch_sync")
+
+    def set_graph(self, enable=True):
+        _check(self._lib.sfac_batch_set_graph(self._b, int(bool(enable))), "batch set_graph")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.sfac_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
 
 
 # ---- the Ewald kernel (ewald.py:124-132) ------------------------------------

@@ -57,8 +57,31 @@ double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6]
 void *sfac_stream(void *h);
 int sfac_sync(void *h);
 
-/* out[0..7] = K0, K1, K2, ldim, order, flags, device bytes held, state */
-int sfac_info(void *h, long long out[8]);
+/* out[0..9] = K0, K1, K2, ldim, order, flags, device bytes held, state,
+ * influence-table mode (0 table in HBM, 1 closed-form Ewald from the
+ * indices), graph replay enabled */
+int sfac_info(void *h, long long out[10]);
+
+/* sfac_eval_dev captures its launches in a hipGraph and replays it while
+ * the arguments stay the same (default on); 0 launches kernel by kernel. */
+int sfac_set_graph(void *h, int enable);
+
+/* Batches: frames (handles with one precision, accumulation mode and spline
+ * order, each with its influence function set) evaluated together, one
+ * launch per stage for all frames, replayed from a captured hipGraph while
+ * the per-frame arguments stay the same.  sfac_batch_set gives frame f its
+ * device pointers (out_d receives E and the virial as in de1_dev);
+ * sfac_batch_eval runs one evaluation of every frame on `stream` (a
+ * hipStream_t; NULL selects the stream of the first frame's handle, which
+ * sfac_batch_sync waits for).  No counterpart in the reference, which
+ * evaluates one system per call (sfac.cpp:190-545). */
+void *sfac_batch_create(void **handles, int nframes);
+int sfac_batch_set(void *b, int f, int n, const double *w_d, const double *x_d, double *out_d,
+                   double *dx_d, double *phi_d);
+int sfac_batch_eval(void *b, void *stream);
+int sfac_batch_sync(void *b);
+int sfac_batch_set_graph(void *b, int enable);
+int sfac_batch_destroy(void *b);
 
 /* Per-kernel timing: while enabled, every launch on the handle's stream is
  * bracketed by HIP events.  sfac_profile_read synchronises, writes up to

@@ -85,3 +85,18 @@ def test_oracle_not_linked_into_product():
             if fn.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text.lower(), fn
+
+
+def test_rad_fn_callback_type_is_accepted(lib):
+    """set_A takes the binding's rad_fn type for every handle (one ctypes
+    type shared by all Sfac objects); a NULL handle reports an error."""
+    from ctypes import c_void_p, cast
+    from ewaldcorrel_amd import ewald
+    for _ in range(2):
+        _, rad_fn, l = ewald.describe_sfac()
+
+        def f(r2, ptr, info):
+            ptr.contents.value = 0.0
+            return 0.0
+
+        assert l.set_A(None, rad_fn(f), 1.0, cast(0, c_void_p)) != 0

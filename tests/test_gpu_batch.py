@@ -1,0 +1,187 @@
+"""Fused evaluation, captured-graph replay, influence-table modes and
+batched frames.  Results must not depend on how a frame is launched: the
+split entry points (sfac/de1/de2/potl), the fused evaluation, its graph
+replay and a batch of uneven frames give the same numbers; in the FP32 and
+fixed-point modes (deterministic int64 spreading, fixed-order reductions)
+they are bitwise identical."""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from conftest import rel, rel_rms  # noqa: E402
+from oracle.sfac_oracle import LofS, OracleSfac, ewald_rad  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ew():
+    from ewaldcorrel_amd import ewald
+    return ewald
+
+
+def make_sys(L6, K, order, N, seed, beta=0.4):
+    rng = np.random.default_rng(seed)
+    L6 = np.asarray(L6, dtype=float)
+    x = rng.uniform(-0.2, 1.2, (N, 3)) @ LofS(L6)
+    w = rng.normal(size=N)
+    w -= w.mean()
+    return dict(L6=L6, K=tuple(K), order=order, beta=beta, w=w, x=x)
+
+
+def split_api(s, w, x):
+    n = len(w)
+    s.sfac(w, x)
+    vir = np.zeros(6)
+    E = s.de1(vir)
+    dx = np.zeros((n, 3))
+    s.de2(n, w, x, dx)
+    phi = np.zeros(n)
+    s.potl(n, x, phi)
+    return E, vir, dx, phi
+
+
+@pytest.mark.parametrize("precision,fixed", [("fp64", False), ("fp64", True), ("fp32", True)])
+def test_split_entry_points_equal_fused_evaluation(precision, fixed):
+    sy = make_sys([20., 18., 22., 3., -2., 1.5], (24, 25, 30), 4, 700, 1)
+    s = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision=precision, fixed=fixed)
+    s.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+    a = split_api(s, sy["w"], sy["x"])
+    b = s.evaluate(sy["w"], sy["x"])
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    say(precision, fixed, abs(a[0] - b[0]) / abs(a[0]), rel(a[1], b[1]), rel_rms(a[2], b[2]))
+    assert abs(a[0] - b[0]) <= tol * abs(a[0])
+    assert rel(a[1], b[1]) <= tol * 10
+    assert rel_rms(a[2], b[2]) <= tol * 10
+    assert rel_rms(a[3], b[3]) <= tol * 10
+
+
+def test_graph_replay_tracks_inputs():
+    """The captured graph is replayed with the same device pointers while
+    the charges and positions in them change; every replay matches a
+    kernel-by-kernel evaluation of the same inputs."""
+    sy = make_sys([16., 17., 18., 0.5, 0.2, -0.3], (20, 24, 30), 6, 900, 2)
+    dev = torch.device("cuda:0")
+    n = len(sy["w"])
+    s = ew().Sfac(sy["L6"], np.array(sy["K"]), 6, precision="fp32")
+    s.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+    ref = ew().Sfac(sy["L6"], np.array(sy["K"]), 6, precision="fp32")
+    ref.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+    ref.set_graph(False)
+    w = torch.tensor(sy["w"], device=dev)
+    x = torch.tensor(sy["x"], device=dev)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+    dx = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    phi = torch.zeros(n, dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(3)
+    for it in range(4):
+        torch.cuda.synchronize()
+        s.eval_dev(n, w.data_ptr(), x.data_ptr(), out.data_ptr(), dx.data_ptr(), phi.data_ptr())
+        s.sync()
+        E, vir, dxr, phr = ref.evaluate(w.cpu().numpy(), x.cpu().numpy())
+        o = out.cpu().numpy()
+        say("replay", it, o[0], E)
+        assert o[0] == E and np.array_equal(o[1:7], vir)
+        assert np.array_equal(dx.cpu().numpy(), dxr)
+        assert np.array_equal(phi.cpu().numpy(), phr)
+        # new inputs, same pointers (program's own eager work between replays)
+        w.mul_(torch.tensor(rng.uniform(0.5, 1.5, n), device=dev))
+        x.add_(torch.tensor(rng.normal(0, 0.3, (n, 3)), device=dev))
+    assert s.info()["graph"] == 1
+
+
+def test_table_modes():
+    """The closed-form Ewald influence is used when it reproduces the full
+    table (fine grid), and the table is read when aliased images matter
+    (coarse grid of the reference's own test); both agree with the oracle."""
+    from ewaldcorrel_amd.workloads import water
+    wsy = water(2048, 40.0, (48, 48, 48), seed=5)
+    s = ew().Sfac(wsy.L6, np.array(wsy.K), 4)
+    s.set_A(ew().ewald_kernel(wsy.beta), 1.0)
+    assert s.info()["table_mode"] == 1
+    E, vir, dx, phi = s.evaluate(wsy.w, wsy.x)
+    o = OracleSfac(wsy.L6, wsy.K, 4)
+    o.set_A(ewald_rad(wsy.beta), 1.0)
+    E0, v0, dx0, phi0 = o.evaluate(wsy.w, wsy.x)
+    say("analytic", abs(E - E0) / abs(E0), rel(vir, v0), rel_rms(dx, dx0))
+    assert abs(E - E0) <= 1e-10 * abs(E0) and rel(vir, v0) <= 1e-10
+    assert rel_rms(dx, dx0) <= 1e-9 and rel_rms(phi, phi0) <= 1e-9
+
+    sy = make_sys([9., 10., 11., 2., -0.1, 1.0], (12, 13, 14), 4, 30, 6, beta=0.5)
+    s = ew().Sfac(sy["L6"], np.array(sy["K"]), 4)
+    s.set_A(ew().ewald_kernel(0.5), 1.0)
+    assert s.info()["table_mode"] == 0
+
+
+def _batch_systems():
+    from ewaldcorrel_amd.workloads import uneven_frames
+    frames = uneven_frames(nframes=12, seed=7, nmin=300, nmax=6000)
+    # a triclinic frame and an odd grid in the mix
+    tri = make_sys([21., 19., 23., 2.5, -1.5, 1.0], (25, 27, 32), 4, 2500, 8, beta=0.32)
+    return frames, tri
+
+
+@pytest.mark.parametrize("precision,fixed", [("fp32", True), ("fp64", True), ("fp64", False)])
+def test_batch_equals_frames_alone(precision, fixed):
+    frames, tri = _batch_systems()
+    dev = torch.device("cuda:0")
+    systems = [dict(L6=f.L6, K=f.K, order=4, beta=f.beta, w=f.w, x=f.x) for f in frames]
+    systems.append(tri)
+    handles, bufs = [], []
+    for sy in systems:
+        h = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision=precision, fixed=fixed)
+        h.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+        handles.append(h)
+        n = len(sy["w"])
+        bufs.append(dict(w=torch.tensor(sy["w"], device=dev), x=torch.tensor(sy["x"], device=dev),
+                         out=torch.zeros(8, dtype=torch.float64, device=dev),
+                         dx=torch.zeros((n, 3), dtype=torch.float64, device=dev),
+                         phi=torch.zeros(n, dtype=torch.float64, device=dev)))
+    # each frame alone first
+    alone = [h.evaluate(sy["w"], sy["x"]) for h, sy in zip(handles, systems)]
+    batch = ew().SfacBatch(handles)
+    for f, (sy, b) in enumerate(zip(systems, bufs)):
+        batch.set(f, len(sy["w"]), b["w"].data_ptr(), b["x"].data_ptr(), b["out"].data_ptr(),
+                  b["dx"].data_ptr(), b["phi"].data_ptr())
+    torch.cuda.synchronize()
+    for rep in range(3):  # graph capture, then replays
+        batch.eval(None)
+        batch.sync()
+        torch.cuda.synchronize()
+        for f, (a, b) in enumerate(zip(alone, bufs)):
+            o = b["out"].cpu().numpy()
+            if fixed:
+                assert o[0] == a[0] and np.array_equal(o[1:7], a[1]), (rep, f)
+                assert np.array_equal(b["dx"].cpu().numpy(), a[2]), (rep, f)
+                assert np.array_equal(b["phi"].cpu().numpy(), a[3]), (rep, f)
+            else:
+                assert abs(o[0] - a[0]) <= 1e-13 * abs(a[0]), (rep, f)
+                assert rel(o[1:7], a[1]) <= 1e-12
+                assert rel_rms(b["dx"].cpu().numpy(), a[2]) <= 1e-12
+    # and the batch agrees with the oracle on two frames
+    for f in (0, len(systems) - 1):
+        sy = systems[f]
+        o = OracleSfac(sy["L6"], sy["K"], 4)
+        o.set_A(ewald_rad(sy["beta"]), 1.0)
+        E0, v0, dx0, phi0 = o.evaluate(sy["w"], sy["x"])
+        E = bufs[f]["out"][0].item()
+        tol = 1e-10 if precision == "fp64" else 1e-6
+        say(precision, f, abs(E - E0) / abs(E0))
+        assert abs(E - E0) <= tol * abs(E0)
+    batch.close()
+
+
+def test_batch_rejects_mixed_precision():
+    a = ew().Sfac(np.array([10., 10, 10, 0, 0, 0]), np.array([16, 16, 16]), 4, precision="fp32")
+    b = ew().Sfac(np.array([10., 10, 10, 0, 0, 0]), np.array([16, 16, 16]), 4)
+    a.set_A(ew().ewald_kernel(0.4), 1.0)
+    b.set_A(ew().ewald_kernel(0.4), 1.0)
+    with pytest.raises(RuntimeError, match="precision"):
+        ew().SfacBatch([a, b])
