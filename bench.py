@@ -1,18 +1,20 @@
 #!/usr/bin/env python3
 """SPME reciprocal-space evaluations/s on one MI355X.
 
-One step = one evaluation (sfac + de1 + de2 + potl: spreading, r2c FFT, /B,
-influence multiply with energy/virial, c2r FFT, force + potential gather)
-of the water_98k workload (32,768 rigid 3-site waters = 98,304 atoms, 100 A
-cube, 128^3 grid, B-spline order 4, beta 0.32), with positions and charges
-already resident in HBM.  Influence-table setup (set_A) is outside the
-timed region, as in the reference's usage.
+One evaluation = sfac + de1 + de2 + potl (spreading, r2c FFT, /B, influence
+multiply with energy/virial, c2r FFT, force + potential gather) of a
+water_98k frame (32,768 rigid 3-site waters = 98,304 atoms, 100 A cube,
+128^3 grid, B-spline order 4, beta 0.32), with positions and charges
+already resident in HBM.  Influence-table setup (set_A) is outside the timed
+region, as in the reference's usage.
 
-Prints ONE JSON line.  `value` is the FP32 engine's evaluations/s; the FP64
-engine is measured in the same run and reported beside it, with the
-per-evaluation latency distribution (p50/p99) of both.
+One step = one batched evaluation of --batch independent frames (each its
+own coordinates), one kernel launch per stage for all frames, replayed from
+a captured hipGraph.  `value` is the FP32 engine's evaluations/s over the
+whole job; the FP64 engine, single-frame latency and per-batch p50/p99 are
+reported beside it.  Prints ONE JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload water_98k]
+    python bench.py [--steps K] [--warmup W] [--batch B]
 """
 from __future__ import annotations
 
@@ -28,28 +30,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-METRIC = ("SPME recip evals/s at 98k atoms, FP32 vs FP64 GPU; p50/p99 batched latency")
+METRIC = "SPME recip evals/s at 98k atoms, FP32 vs FP64 GPU; p50/p99 batched latency"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(name):
+def frames(name, count):
     from ewaldcorrel_amd import workloads as wl
     if name == "water_98k":
-        return wl.water_98k()
+        return [wl.water(32768, 100.0, (128, 128, 128), seed=2 + f, name="water_98k")
+                for f in range(count)]
     if name == "point_charges_1k":
-        return wl.point_charges_1k()
+        return [wl.point_charges_1k(seed=1 + f) for f in range(count)]
     if name == "gaussian_1m":
-        return wl.gaussian_1m()
+        return [wl.gaussian_1m(seed=3 + f) for f in range(count)]
     raise SystemExit(f"unknown workload {name}")
 
 
 def kernel_bytes(kernel, n, K, p):
-    """Algorithmic HBM bytes of one launch: the compulsory reads and writes of
-    that kernel.  p = working-precision bytes of the grid; positions,
-    charges, forces and potentials are doubles at the ABI (8 B each)."""
+    """Algorithmic HBM bytes of one frame in one launch: the compulsory reads
+    and writes of that kernel.  p = working-precision bytes of the grid;
+    positions, charges, forces and potentials are doubles at the ABI."""
     K0, K1, K2 = K
     G = K0 * K1 * K2
     H = K0 * K1 * (K2 // 2 + 1)
@@ -81,28 +84,42 @@ def eval_bytes(n, K, p, pg):
     return spread + fwd + infl + inv + gather
 
 
-def time_engine(sy, precision, steps, warmup, profile):
-    import torch
+class Inputs:
+    """Device-resident inputs and outputs of one frame."""
+
+    def __init__(self, sy, dev):
+        import torch
+        n = sy.n
+        self.n = n
+        self.w = torch.tensor(sy.w, dtype=torch.float64, device=dev)
+        self.x = torch.tensor(sy.x, dtype=torch.float64, device=dev)
+        self.out = torch.zeros(8, dtype=torch.float64, device=dev)
+        self.dx = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        self.phi = torch.zeros(n, dtype=torch.float64, device=dev)
+
+
+def make_handles(systems, precision):
     from ewaldcorrel_amd import ewald
-    dev = torch.device("cuda:0")
-    s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
-    s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
-    n = sy.n
-    w = torch.tensor(sy.w, dtype=torch.float64, device=dev)
-    x = torch.tensor(sy.x, dtype=torch.float64, device=dev)
-    out = torch.zeros(8, dtype=torch.float64, device=dev)
-    dx = torch.zeros((n, 3), dtype=torch.float64, device=dev)
-    phi = torch.zeros(n, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(s.stream)
+    hs = []
+    for sy in systems:
+        s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
+        s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
+        hs.append(s)
+    return hs
+
+
+def time_single(sy, handle, inp, steps, warmup):
+    """One frame per step through sfac_eval_dev (graph replay)."""
+    import torch
+    stream = torch.cuda.ExternalStream(handle.stream)
 
     def step():
-        s.eval_dev(n, w.data_ptr(), x.data_ptr(), out.data_ptr(), dx.data_ptr(),
-                   phi.data_ptr())
+        handle.eval_dev(inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                        inp.dx.data_ptr(), inp.phi.data_ptr())
 
     for _ in range(warmup):
         step()
-    s.sync()
+    handle.sync()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
     torch.cuda.synchronize()
@@ -111,27 +128,66 @@ def time_engine(sy, precision, steps, warmup, profile):
         ev[k][0].record(stream)
         step()
         ev[k][1].record(stream)
-    s.sync()
+    handle.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([a.elapsed_time(b) for a, b in ev])
-    res = dict(precision=precision, wall_s=wall, ms_per_step=wall * 1e3 / steps,
-               evals_per_s=steps / wall, p50_ms=float(np.percentile(lat, 50)),
-               p99_ms=float(np.percentile(lat, 99)), mean_event_ms=float(lat.mean()))
-    res["E"] = float(out[0].item())
-    res["vir"] = out[1:7].cpu().numpy().tolist()
-    res["dx"] = dx.cpu().numpy()
-    res["phi"] = phi.cpu().numpy()
+    return dict(evals_per_s=steps / wall, ms_per_eval=wall * 1e3 / steps,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+
+
+def time_batch(batch, handle0, nframes, steps, warmup, profile):
+    """One step = one evaluation of every frame of the batch."""
+    import torch
+    stream = torch.cuda.ExternalStream(handle0.stream)
+    for _ in range(warmup):
+        batch.eval(None)
+    batch.sync()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        batch.eval(None)
+        ev[k][1].record(stream)
+    batch.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    res = dict(evals_per_s=steps * nframes / wall, ms_per_step=wall * 1e3 / steps,
+               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+               mean_event_ms=float(lat.mean()))
     if profile:
-        s.profile(True)
+        handle0.profile(True)
         for _ in range(steps):
-            step()
-        prof = s.profile_read()
-        s.profile(False)
+            batch.eval(None)
+        prof = handle0.profile_read()
+        handle0.profile(False)
         res["kernels"] = {k: {"total_ms": v[0], "launches": v[1],
                               "avg_us": 1e3 * v[0] / max(v[1], 1)} for k, v in prof.items()}
-    del s
     return res
+
+
+def run_precision(systems, precision, steps, warmup, profile):
+    import torch
+    from ewaldcorrel_amd import ewald
+    dev = torch.device("cuda:0")
+    handles = make_handles(systems, precision)
+    inputs = [Inputs(sy, dev) for sy in systems]
+    torch.cuda.synchronize()
+    single = time_single(systems[0], handles[0], inputs[0], steps, warmup)
+    batch = ewald.SfacBatch(handles)
+    for f, inp in enumerate(inputs):
+        batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                  inp.dx.data_ptr(), inp.phi.data_ptr())
+    bat = time_batch(batch, handles[0], len(handles), steps, warmup, profile)
+    bat["E"] = [float(inp.out[0].item()) for inp in inputs]
+    bat["dx0"] = inputs[0].dx.cpu().numpy()
+    bat["phi0"] = inputs[0].phi.cpu().numpy()
+    batch.close()
+    del handles
+    return single, bat
 
 
 def cpu_baseline(sy, budget_s=15.0):
@@ -150,15 +206,16 @@ def cpu_baseline(sy, budget_s=15.0):
             break
     dt = time.perf_counter() - t0
     return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"{sy.name}: {k} full evaluations (FP64 numpy oracle, "
+            "sample": f"{sy.name}: {k} full evaluations of one frame (FP64 numpy oracle, "
                       f"single thread, {dt:.1f} s; set_A excluded)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--workload", default="water_98k")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -169,38 +226,43 @@ def main():
               "measured here.", file=sys.stderr)
         sys.exit(2)
 
-    sy = workload(args.workload)
-    log(f"workload {sy.name}: n={sy.n} K={sy.K} order={sy.order} beta={sy.beta}")
-    r64 = time_engine(sy, "fp64", args.steps, args.warmup, profile=True)
-    log(f"fp64: {r64['evals_per_s']:.1f} evals/s  p50 {r64['p50_ms']:.3f} ms  "
-        f"p99 {r64['p99_ms']:.3f} ms")
-    r32 = time_engine(sy, "fp32", args.steps, args.warmup, profile=True)
-    log(f"fp32: {r32['evals_per_s']:.1f} evals/s  p50 {r32['p50_ms']:.3f} ms  "
-        f"p99 {r32['p99_ms']:.3f} ms")
+    systems = frames(args.workload, args.batch)
+    sy = systems[0]
+    log(f"workload {sy.name}: n={sy.n} K={sy.K} order={sy.order} beta={sy.beta} "
+        f"batch={args.batch}")
+    s64, b64 = run_precision(systems, "fp64", args.steps, args.warmup, profile=True)
+    log(f"fp64: single {s64['evals_per_s']:.0f} evals/s ({s64['p50_ms']:.3f} ms); "
+        f"batch {b64['evals_per_s']:.0f} evals/s, p50 {b64['p50_ms']:.3f} ms/batch")
+    s32, b32 = run_precision(systems, "fp32", args.steps, args.warmup, profile=True)
+    log(f"fp32: single {s32['evals_per_s']:.0f} evals/s ({s32['p50_ms']:.3f} ms); "
+        f"batch {b32['evals_per_s']:.0f} evals/s, p50 {b32['p50_ms']:.3f} ms/batch")
 
     # FP32 against the FP64 engine on the same inputs
-    eE = abs(r32["E"] - r64["E"]) / abs(r64["E"])
-    eF = float(np.sqrt(np.mean((r32["dx"] - r64["dx"]) ** 2)) /
-               np.sqrt(np.mean(r64["dx"] ** 2)))
-    ePhi = float(np.sqrt(np.mean((r32["phi"] - r64["phi"]) ** 2)) /
-                 np.sqrt(np.mean(r64["phi"] ** 2)))
+    eE = max(abs(a - b) / abs(b) for a, b in zip(b32["E"], b64["E"]))
+    eF = float(np.sqrt(np.mean((b32["dx0"] - b64["dx0"]) ** 2)) /
+               np.sqrt(np.mean(b64["dx0"] ** 2)))
+    ePhi = float(np.sqrt(np.mean((b32["phi0"] - b64["phi0"]) ** 2)) /
+                 np.sqrt(np.mean(b64["phi0"] ** 2)))
 
-    # dominant kernel of the FP32 engine, timed live with HIP events
-    ks = r32["kernels"]
+    # dominant kernel of the batched FP32 engine, timed live with HIP events
+    B = args.batch
+    ks = b32["kernels"]
     dom = max(ks, key=lambda k: ks[k]["total_ms"])
-    B = kernel_bytes(dom, sy.n, sy.K, 4)
+    nb = kernel_bytes(dom, sy.n, sy.K, 4)
+    Bk = nb * B if nb else None
     avg_s = ks[dom]["avg_us"] * 1e-6
-    achieved = B / avg_s / 1e9 if B else None
+    achieved = Bk / avg_s / 1e9 if Bk else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{sy.name}_fp32.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{sy.name}_fp32_batch{B}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get(dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": dom, "algorithmic_bytes": B, "avg_us": ks[dom]["avg_us"]}
-    Be32 = eval_bytes(sy.n, sy.K, 4, 4)  # the FP32 grid is written once, as FP32
-    Be64 = eval_bytes(sy.n, sy.K, 8, 8)
+                "kernel": dom, "algorithmic_bytes": Bk, "avg_us": ks[dom]["avg_us"],
+                "frames_per_launch": B}
+    Be32 = eval_bytes(sy.n, sy.K, 4, 4) * B  # the FP32 grid is written once, as FP32
+    Be64 = eval_bytes(sy.n, sy.K, 8, 8) * B
 
     cpu = None if args.no_cpu else cpu_baseline(sy, args.cpu_budget)
     if cpu:
@@ -208,34 +270,40 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": r32["evals_per_s"],
+        "value": b32["evals_per_s"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["ms_per_step"],
+        "ms_per_step": b32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": sy.name, "atoms": sy.n, "grid": list(sy.K), "order": sy.order,
-                   "beta": sy.beta, "max_m": sy.max_m, "precision": "fp32 (FP64 table, "
-                   "reductions and int64 fixed-point grid accumulation)"},
-        "latency_ms": {"p50": r32["p50_ms"], "p99": r32["p99_ms"]},
-        "fp64": {"value": r64["evals_per_s"], "ms_per_step": r64["ms_per_step"],
-                 "latency_ms": {"p50": r64["p50_ms"], "p99": r64["p99_ms"]},
+                   "beta": sy.beta, "max_m": sy.max_m, "frames_per_step": B,
+                   "precision": "fp32 (FP64 influence, reductions and int64 fixed-point "
+                                "grid accumulation)"},
+        "latency_ms": {"p50": b32["p50_ms"], "p99": b32["p99_ms"], "per": "batch"},
+        "single_frame": {"fp32": {"evals_per_s": s32["evals_per_s"],
+                                  "latency_ms": {"p50": s32["p50_ms"], "p99": s32["p99_ms"]}},
+                         "fp64": {"evals_per_s": s64["evals_per_s"],
+                                  "latency_ms": {"p50": s64["p50_ms"], "p99": s64["p99_ms"]}}},
+        "fp64": {"value": b64["evals_per_s"], "ms_per_step": b64["ms_per_step"],
+                 "latency_ms": {"p50": b64["p50_ms"], "p99": b64["p99_ms"], "per": "batch"},
                  "dtype": "f64",
-                 "eval_roofline_frac": Be64 / (r64["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "speedup_fp32_vs_fp64": r32["evals_per_s"] / r64["evals_per_s"],
-        "fp32_error_vs_fp64_engine": {"energy_rel": eE, "force_rel_rms": eF,
+                 "eval_roofline_frac": Be64 / (b64["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "speedup_fp32_vs_fp64": b32["evals_per_s"] / b64["evals_per_s"],
+        "fp32_error_vs_fp64_engine": {"energy_rel_max": eE, "force_rel_rms": eF,
                                       "potential_rel_rms": ePhi},
-        "eval_roofline": {"algorithmic_bytes": Be32,
-                          "achieved_GBs": Be32 / (r32["mean_event_ms"] * 1e-3) / 1e9,
-                          "frac": Be32 / (r32["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "eval_roofline": {"algorithmic_bytes_per_step": Be32,
+                          "achieved_GBs": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9,
+                          "frac": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "roofline": roofline,
-        "kernels_fp32_avg_us": {k: round(v["avg_us"], 3) for k, v in ks.items()},
-        "kernels_fp64_avg_us": {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()},
+        "kernels_fp32_batch_avg_us": {k: round(v["avg_us"], 3) for k, v in ks.items()},
+        "kernels_fp64_batch_avg_us": {k: round(v["avg_us"], 3)
+                                      for k, v in b64["kernels"].items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

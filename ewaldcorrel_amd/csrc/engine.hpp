@@ -86,6 +86,10 @@ public:
     void profile_enable(bool on);
     void profile_read(std::vector<std::string> &names, std::vector<double> &ms,
                       std::vector<int> &count);
+    bool profiling() const { return prof_; }
+    // bracket a launch on stream st (a batch records into its first frame)
+    void prof_begin_on(const char *name, hipStream_t st);
+    void prof_end_on(hipStream_t st);
 
     // descriptor and launch geometry (used by Batch)
     const FrameDesc &desc() const { return desc_; }

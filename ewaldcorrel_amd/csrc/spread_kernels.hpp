@@ -130,13 +130,21 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
     int carry = 0;
     for (int base = 0; base < ncol; base += 1024 * E) {
         const int i0 = base + (int)threadIdx.x * E;
+        // colcount is zero-padded to a multiple of 1024*E: four unconditional
+        // 16-byte loads per thread, all in flight together
         int v[E];
+        const int4 *src = reinterpret_cast<const int4 *>(F.colcount + i0);
+#pragma unroll
+        for (int e4 = 0; e4 < E / 4; ++e4) {
+            const int4 q = src[e4];
+            v[4 * e4 + 0] = q.x;
+            v[4 * e4 + 1] = q.y;
+            v[4 * e4 + 2] = q.z;
+            v[4 * e4 + 3] = q.w;
+        }
         int s = 0;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            v[e] = (i0 + e < ncol) ? F.colcount[i0 + e] : 0;
-            s += v[e];
-        }
+        for (int e = 0; e < E; ++e) s += v[e];
         part[threadIdx.x] = s;
         __syncthreads();
         // Hillis-Steele inclusive scan over the 1024 thread sums
@@ -149,12 +157,12 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
         int run = carry + (threadIdx.x > 0 ? part[threadIdx.x - 1] : 0);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            if (i0 + e < ncol) {
-                F.colstart[i0 + e] = run;
-                F.colcount[i0 + e] = 0;
-            }
+            if (i0 + e < ncol) F.colstart[i0 + e] = run;
             run += v[e];
         }
+        int4 *dst = reinterpret_cast<int4 *>(F.colcount + i0);
+#pragma unroll
+        for (int e4 = 0; e4 < E / 4; ++e4) dst[e4] = make_int4(0, 0, 0, 0);
         carry += part[1023];
         __syncthreads();
     }
