@@ -77,7 +77,7 @@ struct FrameDesc {
     int *a_col, *a_rank;
     void *s_frac;    // T[3n], fractional offsets in column order
     double *s_w;
-    int *s_cell;     // (cx*K1 + cy)*K2 + cz
+    int *s_cell;     // cx << 20 | cy << 10 | cz
     int *s_perm;     // original atom index
     unsigned long long *maxbits;  // max |w| as bits (fixed point)
     double *fxscale;              // fixed-point scale, written by the scan

@@ -18,6 +18,8 @@
 // fraction f adds w M(f0+i) M(f1+j) M(f2+k) to ((c + n/2 - i) mod K, ...).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace spme {
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(SPME_FRAME_ARGS) {
     sf[3 * pos + 1] = T(f[1]);
     sf[3 * pos + 2] = T(f[2]);
     F.s_w[pos] = F.w[a];
-    F.s_cell[pos] = (c[0] * F.K1 + c[1]) * F.K2 + c[2];
+    F.s_cell[pos] = (c[0] << 20) | (c[1] << 10) | c[2];  // K <= 1024: 10 bits each
     F.s_perm[pos] = a;
 }
 
@@ -290,32 +292,65 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         for (int q = 0; q < UPT; ++q) {
             if (pos[q] < 0) continue;
             const int i = iu[q];
-            const int col = cell[q] / K2;
-            const int cz = cell[q] - col * K2;
-            const int cx = col / K1;
-            const int cy = col - cx * K1;
+            const int cx = cell[q] >> 20;
+            const int cy = (cell[q] >> 10) & 1023;
+            const int cz = cell[q] & 1023;
             const int lx = imod(cx + n / 2 - i, K0) - x0;
             if (lx < 0 || lx >= txr) continue;
-            const double wx = wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i);
             T by[CAP], bz[CAP];
             spline_w<T, ORDER>(sp, n, fy[q], by);
             spline_w<T, ORDER>(sp, n, fz[q], bz);
+            // z and y targets once per unit; one conditional wrap suffices when
+            // the stencil is no longer than the grid, imod otherwise
+            int zt[CAP], yt[CAP];
 #pragma unroll
-            for (int j = 0; j < CAP; ++j) {
-                if (j >= n) break;
-                const int ly = imod(cy + n / 2 - j, K1) - y0;
-                if (ly < 0 || ly >= tyr) continue;
-                const double wxy = wx * (double)by[j];
-                Acc *row = acc + (lx * tyr + ly) * K2;
+            for (int k = 0; k < CAP; ++k) {
+                int z = cz + n / 2 - k;
+                if (K2 >= n) z = z < 0 ? z + K2 : (z >= K2 ? z - K2 : z);
+                else z = imod(z, K2);
+                zt[k] = z;
+                int y = cy + n / 2 - k;
+                if (K1 >= n) y = y < 0 ? y + K1 : (y >= K1 ? y - K1 : y);
+                else y = imod(y, K1);
+                yt[k] = y - y0;
+            }
+            if (std::is_same<T, float>::value && FIXED) {
+                // FP32 path: the scaled x factor (|wx| < 2^62, formed in FP64 since
+                // the scale alone may exceed the FP32 range), then FP32 products
+                const float wx =
+                    (float)(wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i) * scale);
 #pragma unroll
-                for (int k = 0; k < CAP; ++k) {
-                    if (k >= n) break;
-                    const int z = imod(cz + n / 2 - k, K2);
-                    if (FIXED) {
-                        const long long qv = llrint(wxy * (double)bz[k] * scale);
-                        atomicAdd((unsigned long long *)&row[z], (unsigned long long)qv);
-                    } else {
-                        atomicAdd((double *)&row[z], wxy * (double)bz[k]);
+                for (int j = 0; j < CAP; ++j) {
+                    if (j >= n) break;
+                    const int ly = yt[j];
+                    if (ly < 0 || ly >= tyr) continue;
+                    const float wxy = wx * (float)by[j];
+                    Acc *row = acc + (lx * tyr + ly) * K2;
+#pragma unroll
+                    for (int k = 0; k < CAP; ++k) {
+                        if (k >= n) break;
+                        const long long qv = (long long)rintf(wxy * (float)bz[k]);
+                        atomicAdd((unsigned long long *)&row[zt[k]], (unsigned long long)qv);
+                    }
+                }
+            } else {
+                const double wx = wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i);
+#pragma unroll
+                for (int j = 0; j < CAP; ++j) {
+                    if (j >= n) break;
+                    const int ly = yt[j];
+                    if (ly < 0 || ly >= tyr) continue;
+                    const double wxy = wx * (double)by[j];
+                    Acc *row = acc + (lx * tyr + ly) * K2;
+#pragma unroll
+                    for (int k = 0; k < CAP; ++k) {
+                        if (k >= n) break;
+                        if (FIXED) {
+                            const long long qv = llrint(wxy * (double)bz[k] * scale);
+                            atomicAdd((unsigned long long *)&row[zt[k]], (unsigned long long)qv);
+                        } else {
+                            atomicAdd((double *)&row[zt[k]], wxy * (double)bz[k]);
+                        }
                     }
                 }
             }
