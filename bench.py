@@ -65,6 +65,8 @@ def kernel_bytes(kernel, n, K, p):
         "fft_x_recip": 4 * H * p,              # closed-form influence: no table read
         "fft_y_inv": 4 * H * p,
         "fft_z_c2r": 2 * H * p + G * p,
+        "fft_yz_fwd": G * p + 2 * H * p,       # real plane in, half-complex plane out
+        "fft_yz_inv": 2 * H * p + G * p,
         "gather_sorted": G * p + n * (4 * 8 + 4) + n * 4 * 8,
     }
     return table.get(kernel)

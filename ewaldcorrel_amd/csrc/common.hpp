@@ -34,11 +34,34 @@ struct CellArgs {
     double iV;
 };
 
+// Division by a launch-invariant divisor d >= 1 without the integer divide
+// sequence: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1, exact for 0 <= n < 2^31.
+struct FastDiv {
+    unsigned d, m, s;
+};
+inline FastDiv make_fastdiv(unsigned d) {
+    FastDiv f;
+    f.d = d;
+    f.s = 0;
+    while ((1ull << f.s) < d) ++f.s;
+    f.m = (unsigned)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
+    return f;
+}
+__device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
+    return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
+}
+
 struct FftPlanArgs {
     int N;
     int nstages;
     int generic;  // a stage with a radix other than 2,3,4,5,8 (needs two buffers)
     int radix[kMaxRadixStages];
+    // per stage s: N / radix[s], the span Ns (product of earlier radices) and
+    // the twiddle step N / (Ns radix[s])
+    FastDiv dNR[kMaxRadixStages];
+    FastDiv dNs[kMaxRadixStages];
+    int twstep[kMaxRadixStages];
 };
 
 // Coefficients of M_n(x+j), x in [0,1), row j, highest power first.
@@ -87,7 +110,11 @@ struct FrameDesc {
     // FFT line blocking
     int zC;          // complex lines (two real rows each) per z workgroup
     int yC, yT;      // lines per y tile, tiles over ldim
-    int xC, xT;      // lines per x tile, tiles over ldim
+    int xC, xLog;    // x lines per workgroup (2^xLog consecutive (ky, kz) columns)
+    int xB;          // x workgroups per frame
+    FastDiv dldim;   // ldim
+    FastDiv dK2;     // K2
+    FastDiv dpair;   // (K1 + 1) / 2 row pairs of a slab
     // reductions (influence)
     double *partials;
     int nred_i;      // workgroups of the standalone influence launch

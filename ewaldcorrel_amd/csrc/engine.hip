@@ -26,6 +26,7 @@
 #include "gather_kernels.hpp"
 #include "recip_kernels.hpp"
 #include "sfac.hpp"
+#include "slab_kernels.hpp"
 #include "spread_kernels.hpp"
 
 namespace spme {
@@ -43,7 +44,7 @@ const char *stage_name(Stage s) {
         "prep",      "scan",      "scatter",   "spread",    "fft_z_r2c",
         "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
         "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
-        "finish"};
+        "finish",    "fft_yz_fwd", "fft_yz_inv"};
     return names[s];
 }
 
@@ -69,18 +70,23 @@ FftPlanArgs plan_args(const FftPlan &p) {
     memset(&a, 0, sizeof(a));
     a.N = p.N;
     a.nstages = (int)p.radix.size();
+    int Ns = 1;
     for (int s = 0; s < a.nstages; ++s) {
         a.radix[s] = p.radix[s];
         const int R = p.radix[s];
         if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) a.generic = 1;
+        a.dNR[s] = make_fastdiv((unsigned)(p.N / R));
+        a.dNs[s] = make_fastdiv((unsigned)Ns);
+        a.twstep[s] = p.N / (Ns * R);
+        Ns *= R;
     }
     return a;
 }
 
 size_t lds_fft(bool fp32, int N, int C, int generic, bool strided) {
     if (strided)
-        return fp32 ? strided_lds_bytes<float>(N, C, generic)
-                    : strided_lds_bytes<double>(N, C, generic);
+        return fp32 ? ylines_lds_bytes<float>(N, C, generic)
+                    : ylines_lds_bytes<double>(N, C, generic);
     return fp32 ? fft_lds_bytes<float>(N, C, generic) : fft_lds_bytes<double>(N, C, generic);
 }
 
@@ -118,19 +124,19 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             break;
         }
         case ST_ZR2C: fft_z_r2c_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
-        case ST_YF: fft_strided_kernel<T, 1, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0); break;
-        case ST_XF_SCALE:
-            fft_strided_kernel<T, 0, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0);
+        case ST_YF: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
+        case ST_XF_SCALE: fft_x_kernel<T, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
+        case ST_XFUSED: fft_x_kernel<T, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
+        case ST_YZ_FWD:
+            fft_yz_fwd_kernel<T><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
             break;
-        case ST_XFUSED: fft_strided_kernel<T, 0, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,-1, 0); break;
+        case ST_YZ_INV:
+            fft_yz_inv_kernel<T><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
+            break;
         case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
-        case ST_XI:
-            fft_strided_kernel<T, 0, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,+1, finish);
-            break;
-        case ST_YI:
-            fft_strided_kernel<T, 1, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf,+1, finish);
-            break;
+        case ST_XI: fft_x_kernel<T, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, +1, finish); break;
+        case ST_YI: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, +1, finish); break;
         case ST_FINISH: finish_kernel<<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_ZC2R: fft_z_c2r_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_GATHER: gather_kernel<T, ORDER, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf,sp); break;
@@ -156,19 +162,53 @@ void launch_order(int order, Stage s, const Launch &L, int blocks,
 // the reduction of a stage is finalized by the next launch of the sequence:
 // 1 after the fused x pass, 2 after the standalone influence launch
 int finishes(Stage prev, Stage s) {
-    if (prev == ST_XFUSED && s == ST_YI) return 1;
+    if (prev == ST_XFUSED && (s == ST_YI || s == ST_YZ_INV)) return 1;
     if (prev == ST_INFL && s == ST_XI) return 2;
     return 0;
 }
 
-const Stage kSfacStages[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD,
-                             ST_ZR2C, ST_YF,   ST_XF_SCALE};
-const Stage kDe1Stages[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
+const Stage kSfacLines[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE};
+const Stage kSfacSlab[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_YZ_FWD, ST_XF_SCALE};
+const Stage kDe1Lines[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
+const Stage kDe1Slab[] = {ST_INFL, ST_XI, ST_YZ_INV};
 const Stage kEnStages[] = {ST_INFL_E, ST_FINISH};
-const Stage kEvalStages[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C,
-                             ST_YF,   ST_XFUSED, ST_YI,    ST_ZC2R,   ST_GATHER_SORTED};
+const Stage kEvalLines[] = {ST_PREP, ST_SCAN,   ST_SCATTER, ST_SPREAD, ST_ZR2C,
+                            ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_SORTED};
+const Stage kEvalSlab[] = {ST_PREP,   ST_SCAN,   ST_SCATTER, ST_SPREAD,
+                           ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_SORTED};
+
+template <size_t NS> StageSeq seq_of(const Stage (&a)[NS]) { return StageSeq{a, (int)NS}; }
+
+// The slab kernels use up to 160 KB of dynamic LDS: usable when the device
+// allows that much per workgroup and the kernels accept the attribute.
+template <typename T> bool slab_lds_usable(size_t bytes) {
+    int dev = 0, maxsh = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (bytes > (size_t)maxsh) return false;
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        ok = hipFuncSetAttribute((const void *)fft_yz_fwd_kernel<T>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kSlabLds) == hipSuccess &&
+             hipFuncSetAttribute((const void *)fft_yz_inv_kernel<T>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kSlabLds) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+    });
+    return ok;
+}
 
 }  // namespace
+
+StageSeq eval_stages(bool slab) { return slab ? seq_of(kEvalSlab) : seq_of(kEvalLines); }
+StageSeq sfac_stages(bool slab) { return slab ? seq_of(kSfacSlab) : seq_of(kSfacLines); }
+StageSeq de1_stages(bool slab) { return slab ? seq_of(kDe1Slab) : seq_of(kDe1Lines); }
 
 // ---------------------------------------------------------------------------
 Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
@@ -257,20 +297,42 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     const int nrows = K[0] * K[1];
     D.zC = fft_lines(fp32, K[2], (nrows + 1) / 2, D.plan2.generic, false);
     {
-        int c = fft_lines(fp32, K[1], ldim, D.plan1.generic, true);
+        const int c = fft_lines(fp32, K[1], ldim, D.plan1.generic, true);
         D.yT = (ldim + c - 1) / c;
         D.yC = (ldim + D.yT - 1) / D.yT;
-        c = fft_lines(fp32, K[0], ldim, D.plan0.generic, true);
-        D.xT = (ldim + c - 1) / c;
-        D.xC = (ldim + D.xT - 1) / D.xT;
     }
-    for (int d : {0, 1, 2}) {
-        const FftPlanArgs &pa = d == 0 ? D.plan0 : d == 1 ? D.plan1 : D.plan2;
-        const int C = d == 0 ? D.xC : d == 1 ? D.yC : D.zC;
+    for (int d : {1, 2}) {
+        const FftPlanArgs &pa = d == 1 ? D.plan1 : D.plan2;
+        const int C = d == 1 ? D.yC : D.zC;
         const size_t maxel = fp32 ? fft_elems<float>() : fft_elems<double>();
         if (lds_fft(fp32, K[d], C, pa.generic, d != 2) > kLdsBudget || (size_t)C * K[d] > maxel)
             throw std::invalid_argument("grid dimension " + std::to_string(K[d]) +
                                         " does not fit the FFT workgroup");
+    }
+    {
+        // x lines: the largest power of two C <= 64 of (ky, kz) columns with
+        // C K0 elements per workgroup (fewer for the ping-pong transform)
+        const int P = K[1] * ldim;
+        const int cap = D.plan0.generic ? (fp32 ? fft_elems<float>() : fft_elems<double>())
+                                        : kXElems;
+        int lg = 0;
+        while ((2 << lg) <= 64 && (2 << lg) * K[0] <= cap && (1 << lg) < P) ++lg;
+        if ((1 << lg) * K[0] > cap)
+            throw std::invalid_argument("grid dimension " + std::to_string(K[0]) +
+                                        " does not fit the FFT workgroup");
+        D.xLog = lg;
+        D.xC = 1 << lg;
+        D.xB = (P + D.xC - 1) / D.xC;
+    }
+    D.dldim = make_fastdiv((unsigned)ldim);
+    D.dK2 = make_fastdiv((unsigned)K[2]);
+    D.dpair = make_fastdiv((unsigned)((K[1] + 1) / 2));
+    {
+        const size_t lds = fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
+                                : slab_lds_bytes<double>(K[1], K[2], ldim);
+        slab_ok = !D.plan1.generic && !D.plan2.generic && lds <= kSlabLds &&
+                  (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
+        use_slab = slab_ok;
     }
     static const int tiles[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
     for (auto &t : tiles) {
@@ -287,7 +349,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
 
     nred_infl_ = (int)std::min<size_t>((H() + kBlock - 1) / kBlock, 1024);
     D.nred_i = nred_infl_;
-    const int nred_max = std::max(nred_infl_, K[1] * D.xT);
+    const int nred_max = std::max(nred_infl_, D.xB);
     alloc_((void **)&d_desc_, sizeof(FrameDesc));
     memset(&desc_uploaded_, 0xff, sizeof(FrameDesc));
     alloc_((void **)&D.partials, (size_t)nred_max * 8 * sizeof(double));
@@ -398,9 +460,10 @@ void Engine::build_ewald_table(double eta, double max_m) {
     enumerate_ball(c.iL, K, ldim, max_m,
                    [&](long idx, double m2, double mx, double my, double mz) {
                        double f = 0.0, fp = 0.0;
-                       if (!(m2 < 1e-10)) {
-                           f = exp(m2 * fac) / (m2 * M_PI);
-                           fp = (fac - 1.0 / m2) * f;
+                       if (!(m2 < 1e-10)) {  // same expression as ewald_image
+                           const double r = 1.0 / m2;
+                           f = exp(m2 * fac) * r * (1.0 / M_PI);
+                           fp = (fac - r) * f;
                        }
                        A[idx] += 0.5 * f;
                        double *d = dA + 6 * idx;
@@ -496,7 +559,9 @@ int Engine::stage_blocks(Stage s) const {
         case ST_YI: return K[0] * D.yT;
         case ST_XF_SCALE:
         case ST_XFUSED:
-        case ST_XI: return K[1] * D.xT;
+        case ST_XI: return D.xB;
+        case ST_YZ_FWD:
+        case ST_YZ_INV: return K[0];
         case ST_INFL:
         case ST_INFL_E: return nred_infl_;
         case ST_FINISH: return 1;
@@ -519,7 +584,13 @@ size_t Engine::stage_lds(Stage s) const {
         case ST_YI: return lds_fft(fp32, K[1], D.yC, D.plan1.generic, true);
         case ST_XF_SCALE:
         case ST_XFUSED:
-        case ST_XI: return lds_fft(fp32, K[0], D.xC, D.plan0.generic, true);
+        case ST_XI:
+            return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, D.plan0.generic)
+                        : xlines_lds_bytes<double>(K[0], D.xC, D.plan0.generic);
+        case ST_YZ_FWD:
+        case ST_YZ_INV:
+            return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
+                        : slab_lds_bytes<double>(K[1], K[2], ldim);
         default: return 0;
     }
 }
@@ -546,12 +617,11 @@ void Engine::run_(Stage s, int finish) {
     launch(s, Launch{d_desc_, nullptr, 1}, nb, stage_lds(s), stream_, finish);
 }
 
-template <size_t NS>
-void Engine::run_seq_(const Stage (&seq)[NS]) {
+void Engine::run_seq_(StageSeq seq) {
     Stage prev = ST_COUNT;
-    for (Stage s : seq) {
-        run_(s, finishes(prev, s));
-        prev = s;
+    for (int i = 0; i < seq.n; ++i) {
+        run_(seq.s[i], finishes(prev, seq.s[i]));
+        prev = seq.s[i];
     }
 }
 
@@ -559,7 +629,7 @@ void Engine::run_seq_(const Stage (&seq)[NS]) {
 void Engine::spread_forward(int n, const double *w_d, const double *x_d) {
     ensure_atoms(n);
     set_atoms_(n, w_d, x_d, desc_.dx, desc_.phi, desc_.out);
-    run_seq_(kSfacStages);
+    run_seq_(sfac_stages(slab()));
     state = 0;
 }
 
@@ -567,14 +637,14 @@ void Engine::energy(double *out_d) {
     if (state != 0) throw UsageError("You must call sfac before this routine.");
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
     desc_.out = out_d;
-    run_seq_(kEnStages);
+    run_seq_(seq_of(kEnStages));
 }
 
 void Engine::convolve(double *out_d) {
     if (state != 0) throw UsageError("You must call sfac before this routine.");
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
     desc_.out = out_d;
-    run_seq_(kDe1Stages);
+    run_seq_(de1_stages(slab()));
     state = 1;
 }
 
@@ -593,14 +663,14 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
     const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.s_frac};
     const bool graph = use_graph && !prof_;
     if (graph) {
-        const bool same =
-            graph_exec_ && graph_n_ == n && memcmp(key, graph_key_, sizeof(key)) == 0;
+        const bool same = graph_exec_ && graph_n_ == n && graph_slab_ == slab() &&
+                          memcmp(key, graph_key_, sizeof(key)) == 0;
         if (!same) {
             drop_graph_();
             sync_desc_();  // the descriptor upload stays outside the graph
             HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             try {
-                run_seq_(kEvalStages);
+                run_seq_(eval_stages(slab()));
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream_, &g);
@@ -610,11 +680,12 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
             HIPC(hipStreamEndCapture(stream_, &graph_));
             HIPC(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
             graph_n_ = n;
+            graph_slab_ = slab();
             memcpy(graph_key_, key, sizeof(key));
         }
         HIPC(hipGraphLaunch(graph_exec_, stream_));
     } else {
-        run_seq_(kEvalStages);
+        run_seq_(eval_stages(slab()));
     }
     state = 1;
 }
@@ -788,6 +859,8 @@ Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
     const int nf = (int)frames_.size();
     descs_.resize(nf);
     for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
+    slab_ = true;
+    for (const Engine *e : frames_) slab_ = slab_ && e->slab();
     HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
     HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
 }
@@ -849,7 +922,9 @@ void Batch::evaluate(hipStream_t st) {
     Engine *e0 = frames_[0];
     auto run_all = [&]() {
         Stage prev = ST_COUNT;
-        for (Stage s : kEvalStages) {
+        const StageSeq seq = eval_stages(slab_);
+        for (int i = 0; i < seq.n; ++i) {
+            const Stage s = seq.s[i];
             const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));

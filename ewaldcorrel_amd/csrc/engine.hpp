@@ -19,13 +19,24 @@ enum EngineFlags {
     kFlagFixedPoint = 2,  // deterministic int64 fixed-point spreading (always on for FP32)
 };
 
-// Kernel stages.  An evaluation runs PREP..ZR2C, YF, XFUSED, YI, ZC2R,
-// GATHER_SORTED; the split entry points use XF_SCALE / INFL / INFL_E / XI.
+// Kernel stages.  An evaluation runs PREP..SPREAD, YZ_FWD, XFUSED, YZ_INV,
+// GATHER_SORTED (or ZR2C, YF in place of YZ_FWD and YI, ZC2R in place of
+// YZ_INV when the y-z slab does not fit in LDS); the split entry points use
+// XF_SCALE / INFL / INFL_E / XI.
 enum Stage {
     ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
     ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
-    ST_COUNT
+    ST_YZ_FWD, ST_YZ_INV, ST_COUNT
 };
+
+// stage sequences of the entry points for the slab / line transforms
+struct StageSeq {
+    const Stage *s;
+    int n;
+};
+StageSeq eval_stages(bool slab);
+StageSeq sfac_stages(bool slab);
+StageSeq de1_stages(bool slab);
 const char *stage_name(Stage s);
 
 // Frames of one launch: frames[0..nf) in device memory, offs = per-frame
@@ -80,6 +91,11 @@ public:
     hipStream_t stream() const { return stream_; }
     double *result_dev() const { return result_; }
     bool use_graph = true;
+    // y-z slab transforms (one LDS-resident plane per workgroup) when the
+    // plane fits; use_slab = false selects the separate z and y line passes
+    bool slab_ok = false;
+    bool use_slab = false;
+    bool slab() const { return slab_ok && use_slab; }
 
     // Per-kernel timing with HIP events recorded on stream() around every
     // launch while enabled (graph replay is off meanwhile).
@@ -126,7 +142,8 @@ public:
 private:
     void alloc_(void **p, size_t bytes);
     void run_(Stage s, int finish);  // single-frame launch of one stage on stream_
-    template <size_t NS> void run_seq_(const Stage (&seq)[NS]);
+    void run_seq_(StageSeq seq);
+    bool graph_slab_ = false;        // path the captured graph was recorded with
     void drop_graph_();
     void set_atoms_(int n, const double *w, const double *x, double *dx, double *phi,
                     double *out);
@@ -196,6 +213,7 @@ private:
     std::vector<int> total_;                // blocks per stage
     std::vector<size_t> lds_;               // max LDS per stage
     bool dirty_ = true;
+    bool slab_ = false;                     // every frame runs the slab transforms
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

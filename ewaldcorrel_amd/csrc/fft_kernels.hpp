@@ -252,6 +252,81 @@ __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, i
     return cur;
 }
 
+// ---- strided line transforms (slab and x kernels) -----------------------------------------
+// C lines in LDS, line c at buf + c LST, element j of a line at + j EST.
+// Butterfly b of a stage belongs to line c = b mod C, index j = b div C, so
+// adjacent lanes work on adjacent lines: with a line stride that is odd in
+// complex elements (or EST = C-contiguous columns) the lanes of a wave hit
+// distinct LDS banks whatever the stage's index pattern.  The index
+// arithmetic uses FastDivs precomputed per stage.  In place, register staged:
+// every thread reads its butterflies' inputs, barrier, writes the outputs.
+// Requires C N <= EPT NT.
+template <typename T, int R, int EPT, int NT>
+__device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, int LST, int EST,
+                                       int N, int Ns, const FastDiv &dNs, int twstep,
+                                       const cplx<T> *tw, int sign) {
+    constexpr int MAXQ = (EPT + R - 1) / R;
+    const int NR = N / R;
+    const int nb = C * NR;
+    cplx<T> v[MAXQ][R];
+    int dst[MAXQ];
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int bf = threadIdx.x + q * NT;
+        dst[q] = -1;
+        if (bf < nb) {
+            const int j = fdiv(bf, dC);
+            const int c = bf - j * C;
+            const int jq = fdiv(j, dNs);
+            const int k = j - jq * Ns;
+            const cplx<T> *s = buf + c * LST + j * EST;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = s[r * NR * EST];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[q][r] = cmul<T>(v[q][r], twiddle<T>(tw, r * k * twstep, sign));
+            }
+            if (R == 2) bfly2<T>(v[q]);
+            else if (R == 3) bfly3<T>(v[q], sign);
+            else if (R == 4) bfly4<T>(v[q], sign);
+            else if (R == 5) bfly5<T>(v[q], sign);
+            else if (R == 8) bfly8<T>(v[q], sign);
+            dst[q] = c * LST + (j + jq * Ns * (R - 1)) * EST;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns * EST] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// All stages of a plan whose radices are 2, 3, 4, 5 and 8 (plan.generic == 0).
+template <typename T, int EPT, int NT>
+__device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC, int LST,
+                                          int EST, const FftPlanArgs &p, const cplx<T> *tw,
+                                          int sign) {
+    int Ns = 1;
+    for (int s = 0; s < p.nstages; ++s) {
+        const int R = p.radix[s];
+        const FastDiv &dNs = p.dNs[s];
+        const int ts = p.twstep[s];
+        switch (R) {
+            case 2: lstage<T, 2, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 3: lstage<T, 3, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 4: lstage<T, 4, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 5: lstage<T, 5, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            default: lstage<T, 8, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+        }
+        Ns *= R;
+    }
+}
+
 // Twiddles to LDS: all loads first (clamped index), then the LDS writes.
 template <typename T>
 __device__ __forceinline__ void load_twiddles(cplx<T> *twl, const void *tw, int N) {

@@ -26,46 +26,50 @@ namespace spme {
 // 1e-10 <= |m|^2 <= max_m^2; f(z) = exp(fac z)/(pi z), f' = (fac - 1/z) f
 // (ewald.py:124-132).  The engine only selects this mode after checking
 // on the host that it reproduces the full table.
+__host__ __device__ inline void ewald_image(const CellArgs &c, double fac, double max_m2,
+                                            int ix, int iy, int iz, double &a, double *d) {
+    const double nx = ix, ny = iy, nz = iz;
+    const double mx = nx * c.iL00;
+    const double my = nx * c.iL10 + ny * c.iL11;
+    const double mz = nx * c.iL20 + ny * c.iL21 + nz * c.iL22;
+    const double m2 = mx * mx + my * my + mz * mz;
+    if (m2 < 1e-10 || m2 > max_m2) return;
+    const double r = 1.0 / m2;
+    const double f = exp(m2 * fac) * r * (1.0 / M_PI);
+    const double fp = (fac - r) * f;
+    a += f;
+    d[0] += fp * mx * mx;
+    d[1] += fp * my * my;
+    d[2] += fp * mz * mz;
+    d[3] += fp * my * mx;
+    d[4] += fp * mz * mx;
+    d[5] += fp * mz * my;
+}
+
 __host__ __device__ inline void ewald_point_impl(int K0, int K1, int K2, const CellArgs &c,
                                                  double fac, double max_m2, int kx, int ky,
                                                  int kz, double &A, double *dA) {
-    int nxs[2], nys[2], nzs[2];
-    int cx = 1, cy = 1, cz = 1;
-    nxs[0] = kx <= K0 / 2 ? kx : kx - K0;
-    if ((K0 % 2) == 0 && kx == K0 / 2 && K0 > 0) { nxs[1] = -kx; cx = 2; }
-    nys[0] = ky <= K1 / 2 ? ky : ky - K1;
-    if ((K1 % 2) == 0 && ky == K1 / 2 && K1 > 0) { nys[1] = -ky; cy = 2; }
-    nzs[0] = kz;
+    const int nx0 = kx <= K0 / 2 ? kx : kx - K0;
+    const int ny0 = ky <= K1 / 2 ? ky : ky - K1;
+    const bool xny = (K0 % 2) == 0 && kx == K0 / 2 && K0 > 0;
+    const bool yny = (K1 % 2) == 0 && ky == K1 / 2 && K1 > 0;
     const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
-    if (zny && kz > 0) { nzs[1] = -kz; cz = 2; }
+    const bool zn2 = zny && kz > 0;
     const double mult = (kz == 0 || zny) ? 1.0 : 2.0;
-    double a = 0.0, d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
-    for (int ix = 0; ix < cx; ++ix)
-        for (int iy = 0; iy < cy; ++iy)
-            for (int iz = 0; iz < cz; ++iz) {
-                const double nx = nxs[ix], ny = nys[iy], nz = nzs[iz];
-                const double mx = nx * c.iL00;
-                const double my = nx * c.iL10 + ny * c.iL11;
-                const double mz = nx * c.iL20 + ny * c.iL21 + nz * c.iL22;
-                const double m2 = mx * mx + my * my + mz * mz;
-                if (m2 < 1e-10 || m2 > max_m2) continue;
-                const double f = exp(m2 * fac) / (m2 * M_PI);
-                const double fp = (fac - 1.0 / m2) * f;
-                a += f;
-                d0 += fp * mx * mx;
-                d1 += fp * my * my;
-                d2 += fp * mz * mz;
-                d3 += fp * my * mx;
-                d4 += fp * mz * mx;
-                d5 += fp * mz * my;
-            }
+    double a = 0.0, d[6] = {0, 0, 0, 0, 0, 0};
+    ewald_image(c, fac, max_m2, nx0, ny0, kz, a, d);
+    if (xny || yny || zn2) {  // Nyquist planes: the other signed images
+        const int cx = xny ? 2 : 1, cy = yny ? 2 : 1, cz = zn2 ? 2 : 1;
+        for (int ix = 0; ix < cx; ++ix)
+            for (int iy = 0; iy < cy; ++iy)
+                for (int iz = 0; iz < cz; ++iz)
+                    if (ix + iy + iz > 0)
+                        ewald_image(c, fac, max_m2, ix ? -nx0 : nx0, iy ? -ny0 : ny0,
+                                    iz ? -kz : kz, a, d);
+    }
     A = 0.5 * mult * a;
-    dA[0] = mult * d0;
-    dA[1] = mult * d1;
-    dA[2] = mult * d2;
-    dA[3] = mult * d3;
-    dA[4] = mult * d4;
-    dA[5] = mult * d5;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dA[q] = mult * d[q];
 }
 
 __device__ __forceinline__ void table_point(const FrameDesc &F, long h, int kx, int ky, int kz,
@@ -105,18 +109,21 @@ __device__ __forceinline__ void block_partial(const FrameDesc &F, int b, const d
 
 // Sum the nparts partials of the previous launch in a fixed order
 // (deterministic) and write E = iV E_raw and the virial
-// (v_ab + delta_ab E_raw) iV^2 (sfac.cpp:384-395).
+// (v_ab + delta_ab E_raw) iV^2 (sfac.cpp:384-395).  The first kBlock threads
+// of the workgroup do the sums, so the order is the same in workgroups of
+// any size >= kBlock; every thread must reach the barrier.
 __device__ __forceinline__ void finalize_reduction(const FrameDesc &F, int nparts) {
     __shared__ double red2[kBlock / 64][8];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double part[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int q = threadIdx.x; q < nparts; q += blockDim.x)
+    if (threadIdx.x < kBlock)
+        for (int q = threadIdx.x; q < nparts; q += kBlock)
 #pragma unroll
-        for (int c = 0; c < 7; ++c) part[c] += F.partials[q * 8 + c];
+            for (int c = 0; c < 7; ++c) part[c] += F.partials[q * 8 + c];
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
         const double v = wave_sum(part[c]);
-        if (lane == 0) red2[wv][c] = v;
+        if (lane == 0 && wv < kBlock / 64) red2[wv][c] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -176,51 +183,36 @@ __global__ void __launch_bounds__(kBlock) influence_kernel(SPME_FRAME_ARGS) {
     block_partial(F, b, acc);
 }
 
-// ---- strided passes ------------------------------------------------------------------------
-// AXIS 1 (y): lines (x = o, kz) along y, o < K0, element stride ldim.
-// AXIS 0 (x): lines (y = o, kz) along x, o < K1, element stride K1*ldim.
-// A frame's blocks are (o, tile); tile t covers kz in [t ldim/T, (t+1) ldim/T).
-// MODE 0: plain transform with `sign`; with `finish`, workgroup 0 of each
-//         frame first finalizes the reduction of the previous launch
-//         (finish 1: after MODE 2, K1*xT partials; 2: after the standalone
-//         influence, nred_i partials).
-// MODE 1: forward transform, outputs scaled by iB0 iB1 iB2 (S-hat).
-// MODE 2 (x only): forward, /B, influence with reductions, inverse -- the
-//         whole reciprocal-space step of an evaluation in one HBM pass.
-// LDS: twiddles (N), iB0 and the tile's iB2 (N + C doubles, padded to an
-// even count), the lines (C x (N+1)), and a second line buffer for plans
-// with a generic radix.
+// ---- y lines (used when the y-z slab does not fit in LDS) --------------------------------
+// Lines (x = o, kz) along y, o < K0, element stride ldim.  A frame's blocks
+// are (o, tile); tile t covers kz in [t ldim/T, (t+1) ldim/T).  finish 1:
+// workgroup 0 of each frame first finalizes the reduction of the fused x
+// pass that ran before.
 template <typename T>
-__host__ __device__ inline size_t strided_lds_bytes(int N, int C, int generic) {
-    return fft_lds_bytes<T>(N, C, generic) + (size_t)((N + C + 1) & ~1) * sizeof(double);
+__host__ __device__ inline size_t ylines_lds_bytes(int N, int C, int generic) {
+    return fft_lds_bytes<T>(N, C, generic);
 }
 
-template <typename T, int AXIS, int MODE>
-__global__ void __launch_bounds__(kBlock) fft_strided_kernel(SPME_FRAME_ARGS, int sign,
-                                                             int finish) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) fft_y_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int b;
     const FrameDesc &F = frames[find_frame(offs, nf, b)];
     if (finish && b == 0) {
-        finalize_reduction(F, finish == 1 ? F.K1 * F.xT : F.nred_i);
+        finalize_reduction(F, F.xB);
         __syncthreads();
     }
-    const FftPlanArgs &p = AXIS == 1 ? F.plan1 : F.plan0;
-    const void *twg = AXIS == 1 ? F.tw1 : F.tw0;
+    const FftPlanArgs &p = F.plan1;
     const int N = p.N, LS = N + 1, ldim = F.ldim;
-    const int C = AXIS == 1 ? F.yC : F.xC;
-    const int Tt = AXIS == 1 ? F.yT : F.xT;
-    const long ostride = AXIS == 1 ? (long)F.K1 * ldim : (long)ldim;
-    const long estride = AXIS == 1 ? (long)ldim : (long)F.K1 * ldim;
+    const int Tt = F.yT;
+    const long estride = ldim;
     const int o = b / Tt, t = b - o * Tt;
     const int c0 = (int)((long)t * ldim / Tt), c1 = (int)((long)(t + 1) * ldim / Tt);
     const int nc = c1 - c0;
     cplx<T> *twl = reinterpret_cast<cplx<T> *>(smem_raw);
-    double *ib0 = reinterpret_cast<double *>(twl + N);
-    double *ib2 = ib0 + N;
-    cplx<T> *b0 = reinterpret_cast<cplx<T> *>(ib0 + ((N + C + 1) & ~1));
-    cplx<T> *b1 = b0 + C * LS;
-    cplx<T> *base = (cplx<T> *)F.grid + (long)o * ostride + c0;
+    cplx<T> *b0 = twl + N;
+    cplx<T> *b1 = b0 + F.yC * LS;
+    cplx<T> *base = (cplx<T> *)F.grid + (long)o * F.K1 * ldim + c0;
     constexpr int EPT = fft_elems<T>() / kBlock;
     const int tot = nc * N;
     {
@@ -234,11 +226,7 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(SPME_FRAME_ARGS, in
             vx[q] = w.x;
             vy[q] = w.y;
         }
-        load_twiddles<T>(twl, twg, N);
-        if (MODE != 0) {
-            for (int e = threadIdx.x; e < N; e += blockDim.x) ib0[e] = F.iB0[e];
-            for (int c = threadIdx.x; c < nc; c += blockDim.x) ib2[c] = F.iB2[c0 + c];
-        }
+        load_twiddles<T>(twl, F.tw1, N);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int l = threadIdx.x + q * kBlock;
@@ -252,53 +240,148 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(SPME_FRAME_ARGS, in
         }
     }
     __syncthreads();
-    const int s0 = MODE == 0 ? sign : -1;
-    cplx<T> *res = fft_lines<T>(b0, b1, nc, LS, p, twl, s0);
-    if (MODE == 2) {
-        // kx = e, ky = o, kz = c0 + c
-        const bool even = (F.K2 % 2) == 0;
-        const double iby = F.iB1[o];
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-            const int c = l / N, e = l - c * N;
-            const int kz = c0 + c;
-            const double ib = ib0[e] * iby * ib2[c];
-            cplx<T> v = res[c * LS + e];
-            const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
-            const double s = re * re + im * im;
-            const long h = ((long)e * F.K1 + o) * ldim + kz;
-            double A, dA[6];
-            table_point(F, h, e, o, kz, A, dA);
-            acc[0] += A * s;
+    cplx<T> *res = fft_lines<T>(b0, b1, nc, LS, p, twl, sign);
+    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
+        const int e = l / nc, c = l - e * nc;
+        base[(long)e * estride + c] = res[c * LS + e];
+    }
+}
+
+// ---- x lines ------------------------------------------------------------------------------
+// Lines along x (element stride P = K1 ldim complex) for C = 2^xLog
+// consecutive columns f = ky ldim + kz of the (ky, kz) plane: each x element
+// of a workgroup is one contiguous C-element run in HBM.  Thread t owns
+// column c = t mod C for the whole launch (its ky, kz and 1/B factors are
+// computed once) and elements e = t div C + q (256 / C).
+// MODE 0: transform with `sign` (finish 2: workgroup 0 of each frame first
+//         finalizes the standalone influence reduction).
+// MODE 1: forward transform, outputs scaled by iB0 iB1 iB2 (S-hat).
+// MODE 2: forward, /B, influence with reductions, inverse -- the whole
+//         reciprocal-space step of an evaluation in one HBM pass.
+// LDS: twiddles (N), iB0 (N doubles, padded to even), C lines of stride N|1
+// (two sets for a plan with a radix other than 2, 3, 4, 5, 8, which runs
+// the ping-pong line transform with C N <= fft_elems<T>()).
+constexpr int kXElems = 2048;  // complex elements per x workgroup (8 per thread)
+
+__host__ __device__ inline int xline_stride(int N) { return N | 1; }
+
+template <typename T>
+__host__ __device__ inline size_t xlines_lds_bytes(int N, int C, int generic) {
+    return ((size_t)N + (size_t)C * xline_stride(N) * (generic ? 2 : 1)) * sizeof(cplx<T>) +
+           (size_t)((N + 1) & ~1) * sizeof(double);
+}
+
+// x transform of the C lines in LDS; returns the buffer holding the result
+template <typename T>
+__device__ __forceinline__ cplx<T> *x_transform(cplx<T> *buf, cplx<T> *other, int C,
+                                                const FastDiv &dC, int LS, const FftPlanArgs &p,
+                                                const cplx<T> *tw, int sign) {
+    if (p.generic) return fft_lines<T>(buf, other, C, LS, p, tw, sign);
+    lines_fft<T, kXElems / kBlock, kBlock>(buf, C, dC, LS, 1, p, tw, sign);
+    return buf;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int EPT = kXElems / kBlock;
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    if (finish && b == 0) {
+        finalize_reduction(F, F.nred_i);
+        __syncthreads();
+    }
+    const FftPlanArgs &p = F.plan0;
+    const int N = p.N, LS = xline_stride(N), ldim = F.ldim;
+    const int C = F.xC, lg = F.xLog;
+    const int P = F.K1 * ldim;
+    cplx<T> *twl = reinterpret_cast<cplx<T> *>(smem_raw);
+    double *ib0 = reinterpret_cast<double *>(twl + N);
+    cplx<T> *lines = reinterpret_cast<cplx<T> *>(ib0 + ((N + 1) & ~1));
+    cplx<T> *lines2 = lines + C * LS;  // generic plans only
+    const int c = threadIdx.x & (C - 1);
+    const int e0 = threadIdx.x >> lg, es = kBlock >> lg;
+    const int f = b * C + c;          // this thread's column
+    const bool valid = f < P;
+    const int fc = valid ? f : P - 1;  // clamped for loads
+    cplx<T> *col = (cplx<T> *)F.grid + fc;
+    {
+        T vx[EPT], vy[EPT];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) acc[1 + q] += s * dA[q];
-            double tt = A * ib;
-            if (kz == 0 || (even && kz == ldim - 1)) tt *= 2.0;
-            v.x = T(re * tt);
-            v.y = T(im * tt);
-            res[c * LS + e] = v;
+        for (int q = 0; q < EPT; ++q) {
+            const int e = min(e0 + q * es, N - 1);
+            const cplx<T> w = col[(long)e * P];
+            vx[q] = w.x;
+            vy[q] = w.y;
+        }
+        load_twiddles<T>(twl, F.tw0, N);
+        if (MODE != 0)
+            for (int e = threadIdx.x; e < N; e += kBlock) ib0[e] = F.iB0[e];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * es;
+            if (e < N) {
+                cplx<T> w;
+                w.x = vx[q];
+                w.y = vy[q];
+                lines[c * LS + e] = w;
+            }
+        }
+    }
+    __syncthreads();
+    FastDiv dC;
+    dC.d = C;
+    dC.m = 1;  // power of two: n div C = n >> lg
+    dC.s = lg;
+    const int s0 = MODE == 0 ? sign : -1;
+    cplx<T> *res = x_transform<T>(lines, lines2, C, dC, LS, p, twl, s0);
+    const int ky = fdiv(fc, F.dldim), kz = fc - ky * ldim;
+    const double ibyz = (MODE != 0) ? F.iB1[ky] * F.iB2[kz] : 1.0;
+    if (MODE == 2) {
+        const bool even = (F.K2 % 2) == 0;
+        const double two = (kz == 0 || (even && kz == ldim - 1)) ? 2.0 : 1.0;
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * es;
+            if (e < N && valid) {
+                const double ib = ib0[e] * ibyz;
+                cplx<T> v = res[c * LS + e];
+                const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+                const double s = re * re + im * im;
+                double A, dA[6];
+                table_point(F, (long)e * P + f, e, ky, kz, A, dA);
+                acc[0] += A * s;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[1 + k] += s * dA[k];
+                const double tt = A * ib * two;
+                v.x = T(re * tt);
+                v.y = T(im * tt);
+                res[c * LS + e] = v;
+            }
         }
         __syncthreads();
-        // the inverse runs in the buffer that holds the data
-        cplx<T> *other = (res == b0) ? b1 : b0;
-        res = fft_lines<T>(res, other, nc, LS, p, twl, +1);
-        for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-            const int e = l / nc, c = l - e * nc;
-            base[(long)e * estride + c] = res[c * LS + e];
+        res = x_transform<T>(res, res == lines ? lines2 : lines, C, dC, LS, p, twl, +1);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * es;
+            if (e < N && valid) col[(long)e * P] = res[c * LS + e];
         }
         block_partial(F, b, acc);
         return;
     }
-    const double iby = (MODE == 1) ? F.iB1[o] : 1.0;
-    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-        const int e = l / nc, c = l - e * nc;
-        cplx<T> v = res[c * LS + e];
-        if (MODE == 1) {
-            const T s = T(ib0[e] * iby * ib2[c]);
-            v.x *= s;
-            v.y *= s;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = e0 + q * es;
+        if (e < N && valid) {
+            cplx<T> v = res[c * LS + e];
+            if (MODE == 1) {
+                const T s = T(ib0[e] * ibyz);
+                v.x *= s;
+                v.y *= s;
+            }
+            col[(long)e * P] = v;
         }
-        base[(long)e * estride + c] = v;
     }
 }
 

@@ -274,7 +274,14 @@ int sfac_sync(void *h) {
     return guard([&] { as_sfac(h)->engine->sync(); });
 }
 
-int sfac_info(void *h, long long out[10]) {
+int sfac_set_transform(void *h, int mode) {
+    return guard([&] {
+        if (mode != 0 && mode != 1) throw std::invalid_argument("transform mode must be 0 or 1");
+        as_sfac(h)->engine->use_slab = mode == 0;
+    });
+}
+
+int sfac_info(void *h, long long out[12]) {
     return guard([&] {
         spme::Engine *e = as_sfac(h)->engine;
         out[0] = e->K[0];
@@ -287,6 +294,8 @@ int sfac_info(void *h, long long out[10]) {
         out[7] = e->state;
         out[8] = e->table_mode();
         out[9] = e->use_graph ? 1 : 0;
+        out[10] = e->slab() ? 1 : 0;
+        out[11] = e->slab_ok ? 1 : 0;
     });
 }
 

@@ -107,6 +107,7 @@ def describe_sfac():
         int_fn(lib.sfac_sync, sfac_p)
         int_fn(lib.sfac_info, sfac_p, POINTER(c_longlong))
         int_fn(lib.sfac_set_graph, sfac_p, c_int)
+        int_fn(lib.sfac_set_transform, sfac_p, c_int)
         decl_fn(lib.sfac_batch_create, POINTER(c_void_p), c_int, c_void_p)
         int_fn(lib.sfac_batch_set, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
                c_void_p, c_void_p)
@@ -264,11 +265,15 @@ class Sfac:
         """Replay eval_dev/evaluate from a captured hipGraph (default on)."""
         _check(self._libsfac.sfac_set_graph(self._data, int(bool(enable))), "set_graph")
 
+    def set_transform(self, mode):
+        """0: y-z slab transforms where available (default); 1: z and y line passes."""
+        _check(self._libsfac.sfac_set_transform(self._data, int(mode)), "set_transform")
+
     def info(self):
-        out = (c_longlong * 10)()
+        out = (c_longlong * 12)()
         _check(self._libsfac.sfac_info(self._data, out), "info")
         keys = ("K0", "K1", "K2", "ldim", "order", "flags", "device_bytes", "state",
-                "table_mode", "graph")
+                "table_mode", "graph", "slab", "slab_available")
         return dict(zip(keys, list(out)))
 
 

@@ -57,10 +57,15 @@ double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6]
 void *sfac_stream(void *h);
 int sfac_sync(void *h);
 
-/* out[0..9] = K0, K1, K2, ldim, order, flags, device bytes held, state,
+/* out[0..11] = K0, K1, K2, ldim, order, flags, device bytes held, state,
  * influence-table mode (0 table in HBM, 1 closed-form Ewald from the
- * indices), graph replay enabled */
-int sfac_info(void *h, long long out[10]);
+ * indices), graph replay enabled, y-z slab transforms in use, slab
+ * transforms available for this grid */
+int sfac_info(void *h, long long out[12]);
+
+/* Transform path: 0 (default) the y-z slab transforms when the plane fits
+ * in LDS and K1, K2 factor into 2, 3, 5; 1 separate z and y line passes. */
+int sfac_set_transform(void *h, int mode);
 
 /* sfac_eval_dev captures its launches in a hipGraph and replays it while
  * the arguments stay the same (default on); 0 launches kernel by kernel. */
