@@ -212,6 +212,30 @@ def cpu_baseline(sy, budget_s=15.0):
                       f"single thread, {dt:.1f} s; set_A excluded)"}
 
 
+def counter_run(systems, steps):
+    import torch
+    from ewaldcorrel_amd import ewald
+    dev = torch.device("cuda:0")
+    handles = make_handles(systems, "fp32")
+    inputs = [Inputs(sy, dev) for sy in systems]
+    batch = ewald.SfacBatch(handles)
+    for f, inp in enumerate(inputs):
+        batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                  inp.dx.data_ptr(), inp.phi.data_ptr())
+    for _ in range(steps):
+        batch.eval(None)
+    batch.sync()
+    batch.close()
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"counter_run": systems[0].name, "frames": len(systems), "steps": steps,
+                      "copy_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +245,10 @@ def main():
     ap.add_argument("--workload", default="water_98k")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="short run for hardware-counter passes: the batched FP32 "
+                         "evaluation only, then one device copy of 256 MiB whose "
+                         "FETCH_SIZE / WRITE_SIZE calibrate the byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -232,6 +260,9 @@ def main():
     sy = systems[0]
     log(f"workload {sy.name}: n={sy.n} K={sy.K} order={sy.order} beta={sy.beta} "
         f"batch={args.batch}")
+    if args.counters:
+        counter_run(systems, args.steps)
+        return
     s64, b64 = run_precision(systems, "fp64", args.steps, args.warmup, profile=True)
     log(f"fp64: single {s64['evals_per_s']:.0f} evals/s ({s64['p50_ms']:.3f} ms); "
         f"batch {b64['evals_per_s']:.0f} evals/s, p50 {b64['p50_ms']:.3f} ms/batch")

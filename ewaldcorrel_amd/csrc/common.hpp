@@ -74,6 +74,28 @@ enum TableMode {
     kTableAnalytic = 1,  // closed-form Ewald A and dA from the indices (no table)
 };
 
+// One atom in column order, written once by the scatter and read coalesced
+// by the spreading and the sorted gather: fraction f = u - c of u = s K in
+// the working precision, charge, cell c packed as cx << 20 | cy << 10 | cz
+// (K <= 1024), and the atom's index in the caller's arrays.
+template <typename T> struct AtomRec;
+template <> struct __align__(32) AtomRec<float> {
+    float f[3];
+    int cell;
+    double w;
+    int perm;
+    int pad;
+};
+template <> struct __align__(16) AtomRec<double> {
+    double f[3];
+    double w;
+    int cell;
+    int perm;
+    int pad[2];
+};
+static_assert(sizeof(AtomRec<float>) == 32, "AtomRec<float> is 32 bytes");
+static_assert(sizeof(AtomRec<double>) == 48, "AtomRec<double> is 48 bytes");
+
 struct FrameDesc {
     // geometry
     int K0, K1, K2, ldim, order;
@@ -98,14 +120,12 @@ struct FrameDesc {
     int *colcount;   // K0*K1 counters, zero between evaluations
     int *colstart;   // K0*K1 + 1 exclusive prefix
     int *a_col, *a_rank;
-    void *s_frac;    // T[3n], fractional offsets in column order
-    double *s_w;
-    int *s_cell;     // cx << 20 | cy << 10 | cz
-    int *s_perm;     // original atom index
+    void *recs;      // AtomRec<T>[n] in column order
     unsigned long long *maxbits;  // max |w| as bits (fixed point)
     double *fxscale;              // fixed-point scale, written by the scan
     double mmax3;
     int fixed;
+    int fx32;        // FP32 spreading: int32 contributions (|q| < 2^30) into int64 sums
     int tx, ty, ntx, nty;  // spreading tiles
     // FFT line blocking
     int zC;          // complex lines (two real rows each) per z workgroup
@@ -126,8 +146,21 @@ struct FrameDesc {
 #define SPME_FRAME_ARGS \
     const FrameDesc *__restrict__ frames, const int *__restrict__ offs, int nf
 
-__device__ __forceinline__ int find_frame(const int *__restrict__ offs, int nf, int &b) {
-    const int bid = blockIdx.x;
+// Workgroups are handed to the 8 XCDs of the chip round-robin (workgroup i
+// to XCD i mod 8), and each XCD has its own L2.  For kernels whose
+// neighbouring workgroups read the same data, xcd_block gives XCD x the
+// contiguous range of logical workgroups [start_x, start_x + count_x), so
+// the shared lines stay in one L2.  A bijection of [0, nb).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % kXcds, i = b / kXcds;
+    const int q = nb / kXcds, r = nb % kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// frame of logical workgroup bid; b = workgroup index within the frame
+__device__ __forceinline__ int find_frame_of(const int *__restrict__ offs, int nf, int bid,
+                                             int &b) {
     if (nf <= 1) {
         b = bid;
         return 0;
@@ -140,6 +173,9 @@ __device__ __forceinline__ int find_frame(const int *__restrict__ offs, int nf, 
     }
     b = bid - offs[lo];
     return lo;
+}
+__device__ __forceinline__ int find_frame(const int *__restrict__ offs, int nf, int &b) {
+    return find_frame_of(offs, nf, (int)blockIdx.x, b);
 }
 
 // Fractional coordinates wrapped to [0,1): s = wrap(iL^T x) (sfac.hpp:56-62).

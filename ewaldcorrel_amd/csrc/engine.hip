@@ -128,10 +128,16 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_XF_SCALE: fft_x_kernel<T, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
         case ST_XFUSED: fft_x_kernel<T, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
         case ST_YZ_FWD:
-            fft_yz_fwd_kernel<T><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
+            if (L.slab_ept == kSlabEptSmall)
+                fft_yz_fwd_kernel<T, kSlabEptSmall><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
+            else
+                fft_yz_fwd_kernel<T, slab_ept<T>()><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
             break;
         case ST_YZ_INV:
-            fft_yz_inv_kernel<T><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
+            if (L.slab_ept == kSlabEptSmall)
+                fft_yz_inv_kernel<T, kSlabEptSmall><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
+            else
+                fft_yz_inv_kernel<T, slab_ept<T>()><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
             break;
         case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
@@ -193,12 +199,14 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        ok = hipFuncSetAttribute((const void *)fft_yz_fwd_kernel<T>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kSlabLds) == hipSuccess &&
-             hipFuncSetAttribute((const void *)fft_yz_inv_kernel<T>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kSlabLds) == hipSuccess;
+        const void *fns[4] = {(const void *)fft_yz_fwd_kernel<T, kSlabEptSmall>,
+                              (const void *)fft_yz_fwd_kernel<T, slab_ept<T>()>,
+                              (const void *)fft_yz_inv_kernel<T, kSlabEptSmall>,
+                              (const void *)fft_yz_inv_kernel<T, slab_ept<T>()>};
+        ok = true;
+        for (const void *fn : fns)
+            ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kSlabLds) == hipSuccess;
         if (!ok) (void)hipGetLastError();
     });
     return ok;
@@ -293,6 +301,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     D.plan2 = plan_args(plan_[2]);
     D.mmax3 = mmax3;
     D.fixed = fixed ? 1 : 0;
+    D.fx32 = fp32 ? 1 : 0;
     D.tabmode = kTableFull;
     const int nrows = K[0] * K[1];
     D.zC = fft_lines(fp32, K[2], (nrows + 1) / 2, D.plan2.generic, false);
@@ -522,10 +531,8 @@ void Engine::ensure_atoms(int n) {
     alloc_((void **)&phi_d_, cap * sizeof(double));
     alloc_((void **)&desc_.a_col, cap * sizeof(int));
     alloc_((void **)&desc_.a_rank, cap * sizeof(int));
-    alloc_((void **)&desc_.s_frac, 3 * cap * tsz);
-    alloc_((void **)&desc_.s_w, cap * sizeof(double));
-    alloc_((void **)&desc_.s_cell, cap * sizeof(int));
-    alloc_((void **)&desc_.s_perm, cap * sizeof(int));
+    alloc_((void **)&desc_.recs,
+           cap * (fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>)));
     atom_cap_ = cap;
     const size_t need = 5 * cap * sizeof(double);
     if (need > pin_cap_) {
@@ -602,6 +609,10 @@ void Engine::launch(Stage s, const Launch &L, int blocks, size_t lds,
     else launch_order<double>(order, s, L, blocks, lds, st, sp_d_, fixed, finish);
 }
 
+int Engine::slab_ept() const {
+    return slab_ept_for(K[1], ldim, fp32 ? spme::slab_ept<float>() : spme::slab_ept<double>());
+}
+
 void Engine::sync_desc_() {
     if (memcmp(&desc_, &desc_uploaded_, sizeof(FrameDesc)) == 0) return;
     // pageable source: the call returns once the bytes are staged
@@ -614,7 +625,7 @@ void Engine::run_(Stage s, int finish) {
     if (nb <= 0) return;
     sync_desc_();
     ProfScope ps(this, stage_name(s));
-    launch(s, Launch{d_desc_, nullptr, 1}, nb, stage_lds(s), stream_, finish);
+    launch(s, Launch{d_desc_, nullptr, 1, slab_ept()}, nb, stage_lds(s), stream_, finish);
 }
 
 void Engine::run_seq_(StageSeq seq) {
@@ -660,7 +671,7 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
     ensure_atoms(n);
     set_atoms_(n, w_d, x_d, dx_d, phi_d, out_d);
-    const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.s_frac};
+    const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.recs};
     const bool graph = use_graph && !prof_;
     if (graph) {
         const bool same = graph_exec_ && graph_n_ == n && graph_slab_ == slab() &&
@@ -860,7 +871,10 @@ Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
     descs_.resize(nf);
     for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
     slab_ = true;
-    for (const Engine *e : frames_) slab_ = slab_ && e->slab();
+    for (const Engine *e : frames_) {
+        slab_ = slab_ && e->slab();
+        slab_ept_ = std::max(slab_ept_, e->slab_ept());
+    }
     HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
     HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
 }
@@ -925,7 +939,7 @@ void Batch::evaluate(hipStream_t st) {
         const StageSeq seq = eval_stages(slab_);
         for (int i = 0; i < seq.n; ++i) {
             const Stage s = seq.s[i];
-            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf};
+            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
             e0->prof_end_on(st);

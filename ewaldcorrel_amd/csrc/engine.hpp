@@ -45,6 +45,7 @@ struct Launch {
     const FrameDesc *frames;
     const int *offs;
     int nf;
+    int slab_ept;  // elements per thread the slab kernels are compiled for
 };
 
 struct FftPlan {
@@ -96,6 +97,7 @@ public:
     bool slab_ok = false;
     bool use_slab = false;
     bool slab() const { return slab_ok && use_slab; }
+    int slab_ept() const;
 
     // Per-kernel timing with HIP events recorded on stream() around every
     // launch while enabled (graph replay is off meanwhile).
@@ -214,6 +216,7 @@ private:
     std::vector<size_t> lds_;               // max LDS per stage
     bool dirty_ = true;
     bool slab_ = false;                     // every frame runs the slab transforms
+    int slab_ept_ = 0;                      // slab kernel variant (largest frame)
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

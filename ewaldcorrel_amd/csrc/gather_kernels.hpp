@@ -6,8 +6,12 @@
 //   phi     = iV sum prod M Q                                  (= dE/dw)
 // n threads per atom, thread i owns the x-plane i of the stencil (n^2 loads,
 // n consecutive reals per row); the planes are summed through LDS.  With
-// SORTED the workgroups walk the atoms in the column order of the last
-// spreading pass, so neighbouring atoms share grid lines in the caches.
+// SORTED the workgroups walk the column-ordered records of the last
+// spreading pass (coalesced, no re-read of the positions), neighbouring
+// atoms share grid lines, and consecutive workgroups are kept on one XCD so
+// those lines are shared in its L2.  From a record's cell c and fraction
+// f = u - c: ceil(u) = c + 1 and ceil(u) - u = 1 - f for f > 0 (exact in
+// FP64), and ceil(u) = c, 0 for f = 0.
 #pragma once
 
 #include "common.hpp"
@@ -37,7 +41,8 @@ template <typename T, int ORDER, bool SORTED>
 __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     __shared__ double red[kBlock][4];
     int b;
-    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const int bid = SORTED ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const FrameDesc &F = frames[find_frame_of(offs, nf, bid, b)];
     const int n = order_of<ORDER>(F.order);
     constexpr int CAP = Ord<ORDER>::cap;
     const int apb = kBlock / n;
@@ -47,19 +52,42 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
     const bool valid = slot < apb && p < F.n;
     double d0 = 0, d1 = 0, d2 = 0, pp = 0;
     int a = 0;
+    double wa = 0.0;
     if (valid) {
-        a = SORTED ? F.s_perm[p] : p;
-        double s0, s1, s2;
-        scale_wrap(F.cell, F.x[3 * a], F.x[3 * a + 1], F.x[3 * a + 2], s0, s1, s2);
-        const double u0 = s0 * F.K0, u1 = s1 * F.K1, u2 = s2 * F.K2;
-        const double k0 = ceil(u0), k1 = ceil(u1), k2 = ceil(u2);
+        int k0, k1, k2;
+        T t0, t1, t2;
+        if (SORTED) {
+            const AtomRec<T> r = ((const AtomRec<T> *)F.recs)[p];
+            a = r.perm;
+            wa = r.w;
+            const int c0 = r.cell >> 20, c1 = (r.cell >> 10) & 1023, c2 = r.cell & 1023;
+            k0 = r.f[0] > T(0) ? c0 + 1 : c0;
+            k1 = r.f[1] > T(0) ? c1 + 1 : c1;
+            k2 = r.f[2] > T(0) ? c2 + 1 : c2;
+            t0 = r.f[0] > T(0) ? T(1) - r.f[0] : T(0);
+            t1 = r.f[1] > T(0) ? T(1) - r.f[1] : T(0);
+            t2 = r.f[2] > T(0) ? T(1) - r.f[2] : T(0);
+        } else {
+            a = p;
+            wa = F.dx ? F.w[a] : 0.0;
+            double s0, s1, s2;
+            scale_wrap(F.cell, F.x[3 * a], F.x[3 * a + 1], F.x[3 * a + 2], s0, s1, s2);
+            const double u0 = s0 * F.K0, u1 = s1 * F.K1, u2 = s2 * F.K2;
+            const double e0 = ceil(u0), e1 = ceil(u1), e2 = ceil(u2);
+            k0 = (int)e0;
+            k1 = (int)e1;
+            k2 = (int)e2;
+            t0 = T(e0 - u0);
+            t1 = T(e1 - u1);
+            t2 = T(e2 - u2);
+        }
         T by[CAP], bz[CAP], dby[CAP], dbz[CAP], bx[CAP], dbx[CAP];
-        spline_wd<T, ORDER>(sp, n, T(k0 - u0), bx, dbx);
-        spline_wd<T, ORDER>(sp, n, T(k1 - u1), by, dby);
-        spline_wd<T, ORDER>(sp, n, T(k2 - u2), bz, dbz);
-        const int gx = imod((int)k0 - n / 2 + i, F.K0);
-        const int o1 = imod((int)k1 - n / 2, F.K1);
-        const int o2 = imod((int)k2 - n / 2, F.K2);
+        spline_wd<T, ORDER>(sp, n, t0, bx, dbx);
+        spline_wd<T, ORDER>(sp, n, t1, by, dby);
+        spline_wd<T, ORDER>(sp, n, t2, bz, dbz);
+        const int gx = imod(k0 - n / 2 + i, F.K0);
+        const int o1 = imod(k1 - n / 2, F.K1);
+        const int o2 = imod(k2 - n / 2, F.K2);
         const T *grid = (const T *)F.grid;
         const long rs = 2L * F.ldim;
         T s_b = 0, s_db = 0, s_bdz = 0;
@@ -110,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
     }
     const CellArgs &c = F.cell;
     if (F.dx) {
-        const double f = -F.w[a] * c.iV;
+        const double f = -wa * c.iV;
         const double g0 = e0 * f * F.K0, g1 = e1 * f * F.K1, g2 = e2 * f * F.K2;
         F.dx[3 * a + 0] = c.iL00 * g0;
         F.dx[3 * a + 1] = c.iL10 * g0 + c.iL11 * g1;

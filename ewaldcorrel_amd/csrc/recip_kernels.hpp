@@ -12,6 +12,8 @@
 // before the loads, so no fences or atomics are needed).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "fft_kernels.hpp"
 
@@ -68,6 +70,57 @@ __host__ __device__ inline void ewald_point_impl(int K0, int K1, int K2, const C
                                     iz ? -kz : kz, a, d);
     }
     A = 0.5 * mult * a;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dA[q] = mult * d[q];
+}
+
+// FP32 evaluation of the same closed form for the FP32 engine: the wave
+// vector and the ball test in FP64 (so the set of images is that of the
+// FP64 form), then exp, 1/m^2 and the products in FP32.
+__device__ __forceinline__ void ewald_image_f32(const CellArgs &c, float fac, double max_m2,
+                                                int ix, int iy, int iz, float &a, float *d) {
+    const double nx = ix, ny = iy, nz = iz;
+    const double mxd = nx * c.iL00;
+    const double myd = nx * c.iL10 + ny * c.iL11;
+    const double mzd = nx * c.iL20 + ny * c.iL21 + nz * c.iL22;
+    const double m2d = mxd * mxd + myd * myd + mzd * mzd;
+    if (m2d < 1e-10 || m2d > max_m2) return;
+    const float m2 = (float)m2d, mx = (float)mxd, my = (float)myd, mz = (float)mzd;
+    const float r = __builtin_amdgcn_rcpf(m2);
+    const float f = expf(m2 * fac) * r * 0.318309886183790672f;  // 1/pi
+    const float fp = (fac - r) * f;
+    a += f;
+    d[0] += fp * mx * mx;
+    d[1] += fp * my * my;
+    d[2] += fp * mz * mz;
+    d[3] += fp * my * mx;
+    d[4] += fp * mz * mx;
+    d[5] += fp * mz * my;
+}
+
+__device__ __forceinline__ void ewald_point_f32(const FrameDesc &F, int kx, int ky, int kz,
+                                                float &A, float *dA) {
+    const int K0 = F.K0, K1 = F.K1, K2 = F.K2;
+    const int nx0 = kx <= K0 / 2 ? kx : kx - K0;
+    const int ny0 = ky <= K1 / 2 ? ky : ky - K1;
+    const bool xny = (K0 % 2) == 0 && kx == K0 / 2;
+    const bool yny = (K1 % 2) == 0 && ky == K1 / 2;
+    const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
+    const bool zn2 = zny && kz > 0;
+    const float mult = (kz == 0 || zny) ? 1.0f : 2.0f;
+    const float fac = (float)F.ewald_fac;
+    float a = 0.0f, d[6] = {0, 0, 0, 0, 0, 0};
+    ewald_image_f32(F.tcell, fac, F.max_m2, nx0, ny0, kz, a, d);
+    if (xny || yny || zn2) {
+        const int cx = xny ? 2 : 1, cy = yny ? 2 : 1, cz = zn2 ? 2 : 1;
+        for (int ix = 0; ix < cx; ++ix)
+            for (int iy = 0; iy < cy; ++iy)
+                for (int iz = 0; iz < cz; ++iz)
+                    if (ix + iy + iz > 0)
+                        ewald_image_f32(F.tcell, fac, F.max_m2, ix ? -nx0 : nx0,
+                                        iy ? -ny0 : ny0, iz ? -kz : kz, a, d);
+    }
+    A = 0.5f * mult * a;
 #pragma unroll
     for (int q = 0; q < 6; ++q) dA[q] = mult * d[q];
 }
@@ -341,23 +394,47 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         const bool even = (F.K2 % 2) == 0;
         const double two = (kz == 0 || (even && kz == ldim - 1)) ? 2.0 : 1.0;
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
+            // FP32 engine, closed form: FP32 terms, FP64 sums
+            const float ibyzf = (float)ibyz, twof = (float)two;
 #pragma unroll 1
-        for (int q = 0; q < EPT; ++q) {
-            const int e = e0 + q * es;
-            if (e < N && valid) {
-                const double ib = ib0[e] * ibyz;
-                cplx<T> v = res[c * LS + e];
-                const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
-                const double s = re * re + im * im;
-                double A, dA[6];
-                table_point(F, (long)e * P + f, e, ky, kz, A, dA);
-                acc[0] += A * s;
+            for (int q = 0; q < EPT; ++q) {
+                const int e = e0 + q * es;
+                if (e < N && valid) {
+                    const float ib = (float)ib0[e] * ibyzf;
+                    cplx<T> v = res[c * LS + e];
+                    const float re = (float)v.x * ib, im = (float)v.y * ib;  // S-hat
+                    const float s = re * re + im * im;
+                    float A, dA[6];
+                    ewald_point_f32(F, e, ky, kz, A, dA);
+                    acc[0] += (double)(A * s);
 #pragma unroll
-                for (int k = 0; k < 6; ++k) acc[1 + k] += s * dA[k];
-                const double tt = A * ib * two;
-                v.x = T(re * tt);
-                v.y = T(im * tt);
-                res[c * LS + e] = v;
+                    for (int k = 0; k < 6; ++k) acc[1 + k] += (double)(s * dA[k]);
+                    const float tt = A * ib * twof;
+                    v.x = T(re * tt);
+                    v.y = T(im * tt);
+                    res[c * LS + e] = v;
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < EPT; ++q) {
+                const int e = e0 + q * es;
+                if (e < N && valid) {
+                    const double ib = ib0[e] * ibyz;
+                    cplx<T> v = res[c * LS + e];
+                    const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+                    const double s = re * re + im * im;
+                    double A, dA[6];
+                    table_point(F, (long)e * P + f, e, ky, kz, A, dA);
+                    acc[0] += A * s;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) acc[1 + k] += s * dA[k];
+                    const double tt = A * ib * two;
+                    v.x = T(re * tt);
+                    v.y = T(im * tt);
+                    res[c * LS + e] = v;
+                }
             }
         }
         __syncthreads();

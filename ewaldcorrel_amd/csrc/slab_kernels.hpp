@@ -38,6 +38,14 @@ template <typename T> __host__ __device__ constexpr int slab_ept() {
 
 __host__ __device__ inline int slab_rows(int K1) { return (K1 + 1) & ~1; }
 
+// planes of up to 9 x 1024 elements (128 x 65 among them) use a variant
+// compiled for 9 elements per thread: fewer registers held per stage
+constexpr int kSlabEptSmall = 9;
+__host__ __device__ inline int slab_ept_for(int K1, int ldim, int ept_max) {
+    return (size_t)slab_rows(K1) * ldim <= (size_t)kSlabEptSmall * kSlabBlock ? kSlabEptSmall
+                                                                            : ept_max;
+}
+
 template <typename T>
 __host__ __device__ inline size_t slab_lds_bytes(int K1, int K2, int ldim) {
     return ((size_t)K1 + K2 + (size_t)slab_rows(K1) * ldim) * sizeof(cplx<T>);
@@ -50,10 +58,9 @@ __device__ __forceinline__ void slab_twiddles(cplx<T> *twz, cplx<T> *twy, const 
     for (int t = threadIdx.x; t < F.K1; t += kSlabBlock) twy[t] = gy[t];
 }
 
-template <typename T>
+template <typename T, int EPT>
 __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int EPT = slab_ept<T>();
     constexpr int U = 8;  // staging registers per round
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -91,7 +98,7 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS)
     lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, -1);
     // untangle the pairs into rows 2c and 2c+1 (all reads before the barrier)
     {
-        constexpr int Q = (slab_elems<T>() / 2 + kSlabBlock - 1) / kSlabBlock;
+        constexpr int Q = (EPT + 1) / 2;
         const int nk = npair * RS;
         cplx<T> za[Q], zb[Q];
 #pragma unroll
@@ -127,10 +134,9 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS)
 
 // finish 1: workgroup 0 of each frame first finalizes the reduction of the
 // fused x pass that ran before.
-template <typename T>
+template <typename T, int EPT>
 __global__ void __launch_bounds__(kSlabBlock) fft_yz_inv_kernel(SPME_FRAME_ARGS, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int EPT = slab_ept<T>();
     constexpr int U = 8;
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -176,7 +182,7 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_inv_kernel(SPME_FRAME_ARGS,
     lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
     // Z_k = A_k + i B_k and the Hermitian completion (all reads before the barrier)
     {
-        constexpr int Q = (slab_elems<T>() / 2 + kSlabBlock - 1) / kSlabBlock;
+        constexpr int Q = (EPT + 1) / 2;
         const int nk = npair * RS;
         const bool even = (K2 % 2) == 0;
         cplx<T> ra[Q], rb[Q];

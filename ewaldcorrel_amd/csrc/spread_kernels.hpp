@@ -6,7 +6,7 @@
 //            column (one counter atomic), block max |w|
 //   scan     per frame: exclusive prefix of the column counts, counters
 //            cleared for the next call, fixed-point scale fixed
-//   scatter  per atom: SoA records (fraction, w, cell, original index) in
+//   scatter  per atom: one record (fraction, w, cell, original index) in
 //            column order
 //   spread   per x-y tile of full-z columns: every atom of the
 //            (tile + order - 1) neighbourhood adds the part of its stencil
@@ -86,15 +86,21 @@ __device__ __forceinline__ double block_max(double v, double *red) {
     return m;
 }
 
-// scale = 2^(62 - e) with n*max|w|*Mmax^3 < 2^e, so no partial sum of the
-// integer grid can overflow int64; clamped so that it stays finite when
-// every charge is zero.
-__device__ __forceinline__ double fixed_scale(int n, unsigned long long maxbits, double mmax3) {
-    const double bound = fmax((double)n * __longlong_as_double((long long)maxbits) * mmax3,
-                              1e-200);
+// FP64 values: scale = 2^(62 - e) with n*max|w| < 2^e; an atom adds at most
+// |w| to any one grid point in total (the spline weights of each axis sum to
+// one, also when a grid shorter than the stencil wraps it onto itself), so no
+// partial sum of the integer grid can overflow int64.  FP32 values: scale =
+// 2^(30 - e) with max|w|*Mmax^3 < 2^e, so every contribution converts to an
+// int32 (|q| < 2^30, a quantum 2^-30 of the largest one, far below FP32
+// rounding) and a sum of fewer than 2^33 of them fits in int64.  Clamped so
+// that it stays finite when every charge is zero.
+__device__ __forceinline__ double fixed_scale(int n, unsigned long long maxbits, double mmax3,
+                                              int fx32) {
+    const double wmax = __longlong_as_double((long long)maxbits);
+    const double bound = fmax(fx32 ? wmax * mmax3 : (double)n * wmax, 1e-200);
     int e;
     frexp(bound, &e);
-    return ldexp(1.0, 62 - e);
+    return ldexp(1.0, (fx32 ? 30 : 62) - e);
 }
 
 // ---- prep: one thread per atom -------------------------------------------------------
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
     }
     if (threadIdx.x == 0) F.colstart[ncol] = carry;
     if (threadIdx.x == 0 && F.fixed) {
-        *F.fxscale = fixed_scale(F.n, *F.maxbits, F.mmax3);
+        *F.fxscale = fixed_scale(F.n, *F.maxbits, F.mmax3, F.fx32);
         *F.maxbits = 0ull;
     }
 }
@@ -186,13 +192,14 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(SPME_FRAME_ARGS) {
     double f[3];
     atom_cell(F, a, c, f);
     const int pos = F.colstart[F.a_col[a]] + F.a_rank[a];
-    T *sf = (T *)F.s_frac;
-    sf[3 * pos + 0] = T(f[0]);
-    sf[3 * pos + 1] = T(f[1]);
-    sf[3 * pos + 2] = T(f[2]);
-    F.s_w[pos] = F.w[a];
-    F.s_cell[pos] = (c[0] << 20) | (c[1] << 10) | c[2];  // K <= 1024: 10 bits each
-    F.s_perm[pos] = a;
+    AtomRec<T> r;
+    r.f[0] = T(f[0]);
+    r.f[1] = T(f[1]);
+    r.f[2] = T(f[2]);
+    r.w = F.w[a];
+    r.cell = (c[0] << 20) | (c[1] << 10) | c[2];
+    r.perm = a;
+    ((AtomRec<T> *)F.recs)[pos] = r;
 }
 
 // ---- spread: one workgroup per x-y tile -------------------------------------------------------
@@ -253,7 +260,7 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     __syncthreads();
     const int units = seg_pref[nseg] * n;
     const double scale = FIXED ? *F.fxscale : 1.0;
-    const T *sf = (const T *)F.s_frac;
+    const AtomRec<T> *recs = (const AtomRec<T> *)F.recs;
 
     // one unit = (atom, i): the x-plane of its stencil.  UPT units per thread
     // per round: their records are loaded together, then processed.
@@ -281,12 +288,12 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         // unconditional loads (index 0 is always allocated) keep them all in flight
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
-            const int p = pos[q] >= 0 ? pos[q] : 0;
-            cell[q] = F.s_cell[p];
-            fx[q] = sf[3 * p];
-            fy[q] = sf[3 * p + 1];
-            fz[q] = sf[3 * p + 2];
-            wv[q] = F.s_w[p];
+            const AtomRec<T> &r = recs[pos[q] >= 0 ? pos[q] : 0];
+            cell[q] = r.cell;
+            fx[q] = r.f[0];
+            fy[q] = r.f[1];
+            fz[q] = r.f[2];
+            wv[q] = r.w;
         }
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
@@ -315,8 +322,9 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
                 yt[k] = y - y0;
             }
             if (std::is_same<T, float>::value && FIXED) {
-                // FP32 path: the scaled x factor (|wx| < 2^62, formed in FP64 since
-                // the scale alone may exceed the FP32 range), then FP32 products
+                // FP32 path: the scaled x factor (|wx| < 2^30, formed in FP64 since
+                // the scale alone may exceed the FP32 range), FP32 products, int32
+                // contributions
                 const float wx =
                     (float)(wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i) * scale);
 #pragma unroll
@@ -329,8 +337,9 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
 #pragma unroll
                     for (int k = 0; k < CAP; ++k) {
                         if (k >= n) break;
-                        const long long qv = (long long)rintf(wxy * (float)bz[k]);
-                        atomicAdd((unsigned long long *)&row[zt[k]], (unsigned long long)qv);
+                        const int qv = __float2int_rn(wxy * (float)bz[k]);
+                        atomicAdd((unsigned long long *)&row[zt[k]],
+                                  (unsigned long long)(long long)qv);
                     }
                 }
             } else {

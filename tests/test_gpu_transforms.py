@@ -46,6 +46,8 @@ GRIDS = [
     # 147 x-pass partials: their sum (finalized inside the 1024-thread slab
     # workgroup) spans several wavefronts
     ("many_x_blocks", [40., 44., 46., 2., 1., -3.], (48, 96, 96), 4, 3000),
+    # 96 x 97 = 9312 plane elements: the slab variant for large planes
+    ("large_slab_variant", [15., 40., 60., 1., -2., 3.], (16, 96, 192), 4, 1500),
 ]
 
 
