@@ -147,7 +147,7 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_ZC2R: fft_z_c2r_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_GATHER: gather_kernel<T, ORDER, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf,sp); break;
         case ST_GATHER_SORTED:
-            gather_kernel<T, ORDER, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf,sp);
+            gather_kernel<T, ORDER, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf, sp);
             break;
         default: break;
     }
@@ -355,6 +355,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     if (D.tx == 0) throw std::invalid_argument("K2 too large for the spreading tile");
     D.ntx = (K[0] + D.tx - 1) / D.tx;
     D.nty = (K[1] + D.ty - 1) / D.ty;
+
 
     nred_infl_ = (int)std::min<size_t>((H() + kBlock - 1) / kBlock, 1024);
     D.nred_i = nred_infl_;
@@ -572,8 +573,8 @@ int Engine::stage_blocks(Stage s) const {
         case ST_INFL:
         case ST_INFL_E: return nred_infl_;
         case ST_FINISH: return 1;
-        case ST_GATHER:
-        case ST_GATHER_SORTED: {
+        case ST_GATHER_SORTED:
+        case ST_GATHER: {
             const int apb = kBlock / order;
             return (D.n + apb - 1) / apb;
         }
@@ -585,6 +586,7 @@ size_t Engine::stage_lds(Stage s) const {
     const FrameDesc &D = desc_;
     switch (s) {
         case ST_SPREAD: return (size_t)D.tx * D.ty * K[2] * 8;
+
         case ST_ZR2C:
         case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);
         case ST_YF:

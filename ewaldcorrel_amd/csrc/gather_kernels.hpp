@@ -19,6 +19,15 @@
 
 namespace spme {
 
+// four consecutive grid values with element (not vector) alignment
+template <typename T> struct Row4;
+template <> struct Row4<float> {
+    typedef float type __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <> struct Row4<double> {
+    typedef double type __attribute__((ext_vector_type(4), aligned(8)));
+};
+
 template <typename T, int ORDER>
 __device__ __forceinline__ void spline_wd(const SplineArgs<T> &sp, int n, T x, T *b, T *db) {
 #pragma unroll
@@ -90,6 +99,9 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
         const int o2 = imod(k2 - n / 2, F.K2);
         const T *grid = (const T *)F.grid;
         const long rs = 2L * F.ldim;
+        // order 4 with the stencil inside the row: one 16-byte (FP32) or two
+        // (FP64) vector loads per row, element-aligned
+        const bool vec = ORDER == 4 && o2 + 4 <= F.K2;
         T s_b = 0, s_db = 0, s_bdz = 0;
 #pragma unroll
         for (int j = 0; j < CAP; ++j) {
@@ -97,15 +109,28 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
             int gy = o1 + j;
             gy = gy >= F.K1 ? gy - F.K1 * (gy / F.K1) : gy;
             const T *row = grid + ((long)gx * F.K1 + gy) * rs;
+            T q[CAP];
+            if (vec) {
+                const typename Row4<T>::type v = *(const typename Row4<T>::type *)(row + o2);
+                q[0] = v.x;
+                q[1] = v.y;
+                q[2] = v.z;
+                q[3] = v.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < CAP; ++k) {
+                    if (k >= n) break;
+                    int z = o2 + k;
+                    z = z >= F.K2 ? z - F.K2 * (z / F.K2) : z;
+                    q[k] = row[z];
+                }
+            }
             T zb = 0, zdb = 0;
 #pragma unroll
             for (int k = 0; k < CAP; ++k) {
                 if (k >= n) break;
-                int z = o2 + k;
-                z = z >= F.K2 ? z - F.K2 * (z / F.K2) : z;
-                const T q = row[z];
-                zb += bz[k] * q;
-                zdb += dbz[k] * q;
+                zb += bz[k] * q[k];
+                zdb += dbz[k] * q[k];
             }
             s_b += by[j] * zb;
             s_db += dby[j] * zb;
