@@ -15,6 +15,15 @@ whole job; the FP64 engine, single-frame latency and per-batch p50/p99 are
 reported beside it.  Prints ONE JSON line.
 
     python bench.py [--steps K] [--warmup W] [--batch B]
+
+HBM traffic of the dominant kernel (roofline.traffic) comes from two
+rocprofv3 counter passes over `bench.py --counters` (FETCH_SIZE, then
+WRITE_SIZE, CSV output), reduced to calibrated bytes per launch by
+
+    python bench.py --traffic FETCH.csv WRITE.csv
+
+which writes profiles/traffic_<workload>_fp32_batch<B>.json.  The counters
+are scaled by what they read for the 256 MiB device copy of the same run.
 """
 from __future__ import annotations
 
@@ -100,12 +109,17 @@ class Inputs:
         self.phi = torch.zeros(n, dtype=torch.float64, device=dev)
 
 
+SET_A_MS = []  # wall time of each influence-table build (device builder + check)
+
+
 def make_handles(systems, precision):
     from ewaldcorrel_amd import ewald
     hs = []
     for sy in systems:
         s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
+        t0 = time.perf_counter()
         s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
+        SET_A_MS.append(1e3 * (time.perf_counter() - t0))
         hs.append(s)
     return hs
 
@@ -184,6 +198,7 @@ def run_precision(systems, precision, steps, warmup, profile):
         batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
                   inp.dx.data_ptr(), inp.phi.data_ptr())
     bat = time_batch(batch, handles[0], len(handles), steps, warmup, profile)
+    bat["table_mode"] = handles[0].info()["table_mode"]
     bat["E"] = [float(inp.out[0].item()) for inp in inputs]
     bat["dx0"] = inputs[0].dx.cpu().numpy()
     bat["phi0"] = inputs[0].phi.cpu().numpy()
@@ -210,6 +225,56 @@ def cpu_baseline(sy, budget_s=15.0):
     return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
             "sample": f"{sy.name}: {k} full evaluations of one frame (FP64 numpy oracle, "
                       f"single thread, {dt:.1f} s; set_A excluded)"}
+
+
+# kernel symbol prefix -> stage name of the batched FP32 evaluation
+STAGE_OF_KERNEL = {
+    "spme::prep_kernel": "prep",
+    "spme::scan_kernel": "scan",
+    "spme::scatter_kernel": "scatter",
+    "spme::spread_tile_kernel": "spread",
+    "spme::fft_yz_fwd_kernel": "fft_yz_fwd",
+    "spme::fft_z_r2c_kernel": "fft_z_r2c",
+    "spme::fft_y_kernel": "fft_y",
+    "spme::fft_x_kernel<float, 2>": "fft_x_recip",
+    "spme::fft_yz_inv_kernel": "fft_yz_inv",
+    "spme::fft_z_c2r_kernel": "fft_z_c2r",
+    "spme::gather_kernel": "gather_sorted",
+}
+
+
+def traffic_from_counters(fetch_csv, write_csv, copy_bytes=256 << 20):
+    """Per-stage HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch), each scaled by counter-read / true bytes of the
+    largest device copy kernel in the same run."""
+    import csv
+    import collections
+
+    def load(path):
+        per = collections.defaultdict(list)
+        copy = 0.0
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            v = float(r["Counter_Value"]) * 1024.0
+            if "copyBuffer" in name:
+                copy = max(copy, v)
+            for pre, st in STAGE_OF_KERNEL.items():
+                if name.startswith(pre) and "<double" not in name:
+                    per[st].append(v)
+        return per, copy
+
+    fetch, cf = load(fetch_csv)
+    write, cw = load(write_csv)
+    ff, fw = copy_bytes / cf, copy_bytes / cw
+    out = {"_calibration": {"copy_bytes": copy_bytes, "fetch_counter_bytes": cf,
+                            "write_counter_bytes": cw, "fetch_factor": ff, "write_factor": fw}}
+    for st in fetch:
+        if st in write:
+            f = ff * sum(fetch[st]) / len(fetch[st])
+            w = fw * sum(write[st]) / len(write[st])
+            out[st] = f + w
+            out["_" + st] = {"fetch_bytes": f, "write_bytes": w, "dispatches": len(fetch[st])}
+    return out
 
 
 def counter_run(systems, steps):
@@ -245,6 +310,9 @@ def main():
     ap.add_argument("--workload", default="water_98k")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="reduce two rocprofv3 counter passes of --counters to the "
+                         "per-stage traffic file read by the roofline")
     ap.add_argument("--counters", action="store_true",
                     help="short run for hardware-counter passes: the batched FP32 "
                          "evaluation only, then one device copy of 256 MiB whose "
@@ -255,6 +323,15 @@ def main():
               "frames and requests shard across GPUs with no exchange, which is not "
               "measured here.", file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic:
+        out = traffic_from_counters(*args.traffic)
+        path = os.path.join(ROOT, "profiles",
+                            f"traffic_{args.workload}_fp32_batch{args.batch}.json")
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        log(f"wrote {path}")
+        return
 
     systems = frames(args.workload, args.batch)
     sy = systems[0]
@@ -316,8 +393,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": sy.name, "atoms": sy.n, "grid": list(sy.K), "order": sy.order,
                    "beta": sy.beta, "max_m": sy.max_m, "frames_per_step": B,
-                   "precision": "fp32 (FP64 influence, reductions and int64 fixed-point "
-                                "grid accumulation)"},
+                   "precision": "fp32 (FP64 coordinates and reductions, int64 fixed-point "
+                                "grid accumulation)",
+                   "influence": ("closed form from the indices (no table read)"
+                                 if b32["table_mode"] == 1 else "table read from HBM")},
+        "set_A_ms": {"median": float(np.median(SET_A_MS)), "max": float(np.max(SET_A_MS)),
+                     "what": "influence table built on the device, checked against "
+                             "the closed form; per frame"},
         "latency_ms": {"p50": b32["p50_ms"], "p99": b32["p99_ms"], "per": "batch"},
         "single_frame": {"fp32": {"evals_per_s": s32["evals_per_s"],
                                   "latency_ms": {"p50": s32["p50_ms"], "p99": s32["p99_ms"]}},

@@ -17,10 +17,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <mutex>
-#include <set>
 #include <type_traits>
 
-#include "ball.hpp"
 #include "engine.hpp"
 #include "fft_kernels.hpp"
 #include "gather_kernels.hpp"
@@ -28,6 +26,9 @@
 #include "sfac.hpp"
 #include "slab_kernels.hpp"
 #include "spread_kernels.hpp"
+#include "table_kernels.hpp"
+#include "correl_kernels.hpp"
+#include "real_kernels.hpp"
 
 namespace spme {
 
@@ -439,83 +440,62 @@ void Engine::upload_table(const double *A, const double *dA) {
     drop_graph_();
 }
 
-// Exact table by the reference's enumeration (every image in the ball),
-// then a check that the closed form evaluated from the indices reproduces
-// it; if so the kernels evaluate it on the fly and no table is read.
+// The exact table (every image of the reference's ball, folded as the
+// reference folds it) is built on the device, one thread per half-complex
+// point, which also compares it with the closed form evaluated from the
+// indices.  If they agree to 1e-13 of the mean magnitude, the kernels
+// evaluate the closed form on the fly and no table is read; otherwise they
+// read the table.
 void Engine::build_ewald_table(double eta, double max_m) {
-    const double fac0 = -(M_PI / eta) * (M_PI / eta);
-    // boxes, grids and kernels already shown to match their closed form
-    static std::mutex mu;
-    static std::set<std::vector<double>> verified;
-    const std::vector<double> key = {L6[0], L6[1], L6[2], L6[3], L6[4], L6[5],
-                                     (double)K[0], (double)K[1], (double)K[2], eta, max_m};
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        if (verified.count(key)) {
-            desc_.tabmode = kTableAnalytic;
-            desc_.ewald_fac = fac0;
-            desc_.max_m2 = max_m * max_m;
-            desc_.tcell = cell;
-            has_table_ = true;
-            drop_graph_();
-            return;
-        }
-    }
     const size_t h = H();
-    std::vector<double> tab(7 * h, 0.0);
-    double *A = tab.data();
-    double *dA = A + h;
-    Cell c(L6);
+    if (!table_) alloc_((void **)&table_, 7 * h * sizeof(double));
     const double fac = -(M_PI / eta) * (M_PI / eta);
-    enumerate_ball(c.iL, K, ldim, max_m,
-                   [&](long idx, double m2, double mx, double my, double mz) {
-                       double f = 0.0, fp = 0.0;
-                       if (!(m2 < 1e-10)) {  // same expression as ewald_image
-                           const double r = 1.0 / m2;
-                           f = exp(m2 * fac) * r * (1.0 / M_PI);
-                           fp = (fac - r) * f;
-                       }
-                       A[idx] += 0.5 * f;
-                       double *d = dA + 6 * idx;
-                       d[0] += fp * mx * mx;
-                       d[1] += fp * my * my;
-                       d[2] += fp * mz * mz;
-                       d[3] += fp * my * mx;
-                       d[4] += fp * mz * mx;
-                       d[5] += fp * mz * my;
-                   });
-    double sumA = 0.0, sumD = 0.0, errA = 0.0, errD = 0.0;
-    for (int kx = 0; kx < K[0]; ++kx)
-        for (int ky = 0; ky < K[1]; ++ky)
-            for (int kz = 0; kz < ldim; ++kz) {
-                const size_t p = ((size_t)kx * K[1] + ky) * ldim + kz;
-                double a, d[6];
-                ewald_point_impl(K[0], K[1], K[2], cell, fac, max_m * max_m, kx, ky, kz, a, d);
-                sumA += fabs(A[p]);
-                errA = std::max(errA, fabs(a - A[p]));
-                double dm = 0.0;
-                for (int q = 0; q < 6; ++q) {
-                    dm = std::max(dm, fabs(dA[6 * p + q]));
-                    errD = std::max(errD, fabs(d[q] - dA[6 * p + q]));
-                }
-                sumD += dm;
-            }
+    BallArgs a;
+    a.K0 = K[0];
+    a.K1 = K[1];
+    a.K2 = K[2];
+    a.ldim = ldim;
+    a.cell = cell;
+    a.R2 = max_m * max_m;
+    a.fac = fac;
+    a.i0 = (int)(max_m / cell.iL00);
+    const int nb = (int)((h + kBlock - 1) / kBlock);
+    if ((size_t)nb * 4 > table_stats_cap_) {
+        alloc_((void **)&table_stats_, (size_t)nb * 4 * sizeof(double));
+        table_stats_cap_ = (size_t)nb * 4;
+    }
+    ewald_table_kernel<<<nb, kBlock, 0, stream_>>>(a, table_, table_ + h, table_stats_);
+    HIPC(hipGetLastError());
+    std::vector<double> st((size_t)nb * 4);
+    HIPC(hipMemcpyAsync(st.data(), table_stats_, st.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    double sumA = 0.0, errA = 0.0, sumD = 0.0, errD = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        sumA += st[4 * b];
+        errA = std::max(errA, st[4 * b + 1]);
+        sumD += st[4 * b + 2];
+        errD = std::max(errD, st[4 * b + 3]);
+    }
     const double tol = 1e-13;
     const bool analytic = errA <= tol * sumA / (double)h && errD <= tol * sumD / (double)h;
-    if (analytic) {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            verified.insert(key);
-        }
-        desc_.tabmode = kTableAnalytic;
-        desc_.ewald_fac = fac;
-        desc_.max_m2 = max_m * max_m;
-        desc_.tcell = cell;
-        has_table_ = true;
-        drop_graph_();
-    } else {
-        upload_table(A, dA);
-    }
+    desc_.table = table_;
+    desc_.tabmode = analytic ? kTableAnalytic : kTableFull;
+    desc_.ewald_fac = fac;
+    desc_.max_m2 = max_m * max_m;
+    desc_.tcell = cell;
+    has_table_ = true;
+    drop_graph_();
+}
+
+void Engine::copy_table(double *A, double *dA) {
+    if (!table_ || !has_table_) throw UsageError("No convolution kernel has been set.");
+    const size_t h = H();
+    if (A) HIPC(hipMemcpyAsync(A, table_, h * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (dA)
+        HIPC(hipMemcpyAsync(dA, table_ + h, 6 * h * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    HIPC(hipStreamSynchronize(stream_));
 }
 
 void Engine::ensure_atoms(int n) {
@@ -799,6 +779,139 @@ double *Engine::get_S_host() {
         HIPC(hipStreamSynchronize(stream_));
     }
     return host_grid_.data();
+}
+
+// ---- structure-factor accumulation ------------------------------------------------------
+void Engine::accum_S2(double *acc_d) {
+    if (state != 0) throw UsageError("You must call sfac before this routine.");
+    const long h = (long)H();
+    const int nb = (int)((h + kBlock - 1) / kBlock);
+    if (fp32) accum_s2_kernel<float><<<nb, kBlock, 0, stream_>>>((const float2 *)grid_, h, acc_d);
+    else accum_s2_kernel<double><<<nb, kBlock, 0, stream_>>>((const double2 *)grid_, h, acc_d);
+    HIPC(hipGetLastError());
+}
+
+void Engine::accum_cross(const Engine &b, double *acc_d) {
+    if (state != 0 || b.state != 0) throw UsageError("You must call sfac before this routine.");
+    if (b.K[0] != K[0] || b.K[1] != K[1] || b.K[2] != K[2] || b.fp32 != fp32)
+        throw std::invalid_argument("cross terms need the same grid and precision");
+    // b's S-hat must be complete before this stream reads it
+    hipEvent_t ev;
+    HIPC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPC(hipEventRecord(ev, b.stream_));
+    HIPC(hipStreamWaitEvent(stream_, ev, 0));
+    HIPC(hipEventDestroy(ev));
+    const long h = (long)H();
+    const int nb = (int)((h + kBlock - 1) / kBlock);
+    if (fp32)
+        accum_cross_kernel<float><<<nb, kBlock, 0, stream_>>>((const float2 *)grid_,
+                                                             (const float2 *)b.grid_, h, acc_d);
+    else
+        accum_cross_kernel<double><<<nb, kBlock, 0, stream_>>>(
+            (const double2 *)grid_, (const double2 *)b.grid_, h, acc_d);
+    HIPC(hipGetLastError());
+}
+
+void Engine::radial(const double *val_d, int nbins, double dk, double *sum_d, double *cnt_d) {
+    if (nbins < 1 || !(dk > 0.0)) throw std::invalid_argument("radial: nbins >= 1 and dk > 0");
+    const size_t h = H();
+    const std::vector<double> key = {L6[0], L6[1], L6[2], L6[3], L6[4], L6[5], (double)nbins, dk};
+    if (key != radial_key_) {
+        // bin of every point, then a stable counting sort by bin
+        std::vector<int> bin(h);
+        std::vector<int> off((size_t)nbins + 1, 0);
+        for (int kx = 0; kx < K[0]; ++kx)
+            for (int ky = 0; ky < K[1]; ++ky)
+                for (int kz = 0; kz < ldim; ++kz) {
+                    const double nx = kx <= K[0] / 2 ? kx : kx - K[0];
+                    const double ny = ky <= K[1] / 2 ? ky : ky - K[1];
+                    const double mx = nx * cell.iL00;
+                    const double my = nx * cell.iL10 + ny * cell.iL11;
+                    const double mz = nx * cell.iL20 + ny * cell.iL21 + kz * cell.iL22;
+                    const double k = 2.0 * M_PI * sqrt(mx * mx + my * my + mz * mz);
+                    const double bf = floor(k / dk);
+                    const int b = bf < (double)nbins ? (int)bf : -1;
+                    const size_t p = ((size_t)kx * K[1] + ky) * ldim + kz;
+                    bin[p] = b;
+                    if (b >= 0) off[(size_t)b + 1] += 1;
+                }
+        for (int b = 0; b < nbins; ++b) off[(size_t)b + 1] += off[b];
+        std::vector<int> perm(std::max<size_t>(off[nbins], 1));
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (size_t p = 0; p < h; ++p)
+            if (bin[p] >= 0) perm[pos[bin[p]]++] = (int)p;
+        if (!radial_perm_) alloc_((void **)&radial_perm_, h * sizeof(int));
+        if ((size_t)nbins + 1 > radial_off_cap_) {
+            alloc_((void **)&radial_off_, ((size_t)nbins + 1) * sizeof(int));
+            radial_off_cap_ = (size_t)nbins + 1;
+        }
+        HIPC(hipMemcpyAsync(radial_perm_, perm.data(), perm.size() * sizeof(int),
+                            hipMemcpyHostToDevice, stream_));
+        HIPC(hipMemcpyAsync(radial_off_, off.data(), off.size() * sizeof(int),
+                            hipMemcpyHostToDevice, stream_));
+        HIPC(hipStreamSynchronize(stream_));
+        radial_key_ = key;
+    }
+    radial_kernel<<<(nbins + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(
+        val_d, radial_perm_, radial_off_, nbins, sum_d, cnt_d);
+    HIPC(hipGetLastError());
+}
+
+// ---- real-space Ewald ----------------------------------------------------------------------
+void Engine::real_space(int n, const double *w_d, const double *x_d, double eta, double rc,
+                        double *out_d, double *dx_d, double *phi_d) {
+    if (n < 0) throw std::invalid_argument("n < 0");
+    if (!(eta > 0.0) || !(rc > 0.0)) throw std::invalid_argument("eta and rc must be > 0");
+    RealArgs a;
+    a.cell = cell;
+    a.L00 = L6[0];
+    a.L10 = L6[3];
+    a.L11 = L6[1];
+    a.L20 = L6[4];
+    a.L21 = L6[5];
+    a.L22 = L6[2];
+    a.eta = eta;
+    a.rc2 = rc * rc;
+    // |s_d + n_d| <= rc |column d of iL| for an image within rc, |s_d| <= 1/2
+    const double c0 = sqrt(cell.iL00 * cell.iL00 + cell.iL10 * cell.iL10 + cell.iL20 * cell.iL20);
+    const double c1 = sqrt(cell.iL11 * cell.iL11 + cell.iL21 * cell.iL21);
+    const double c2 = fabs(cell.iL22);
+    a.e0 = (int)ceil(rc * c0 + 0.5);
+    a.e1 = (int)ceil(rc * c1 + 0.5);
+    a.e2 = (int)ceil(rc * c2 + 0.5);
+    a.n = n;
+    const int nb = std::max(1, (n + kBlock - 1) / kBlock);
+    if ((size_t)nb * 10 > real_parts_cap_) {
+        alloc_((void **)&real_parts_, (size_t)nb * 10 * sizeof(double));
+        real_parts_cap_ = (size_t)nb * 10;
+    }
+    real_space_kernel<<<nb, kBlock, 0, stream_>>>(a, w_d, x_d, dx_d, phi_d, real_parts_);
+    HIPC(hipGetLastError());
+    real_space_finish<<<1, kBlock, 0, stream_>>>(a, nb, real_parts_, phi_d, out_d);
+    HIPC(hipGetLastError());
+}
+
+void Engine::real_space_host(int n, const double *w, const double *x, double eta, double rc,
+                             double out[10], double *dx, double *phi) {
+    if (n < 0) throw std::invalid_argument("n < 0");
+    ensure_atoms(n);
+    if (!real_out_) alloc_((void **)&real_out_, 16 * sizeof(double));
+    HIPC(hipStreamSynchronize(stream_));
+    memcpy(pin_, x, 3 * (size_t)n * sizeof(double));
+    memcpy(pin_ + 3 * (size_t)n, w, (size_t)n * sizeof(double));
+    HIPC(hipMemcpyAsync(x_d_, pin_, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                        stream_));
+    HIPC(hipMemcpyAsync(w_d_, pin_ + 3 * (size_t)n, (size_t)n * sizeof(double),
+                        hipMemcpyHostToDevice, stream_));
+    real_space(n, w_d_, x_d_, eta, rc, real_out_, dx ? dx_d_ : nullptr, phi ? phi_d_ : nullptr);
+    HIPC(hipMemcpyAsync(out, real_out_, 10 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (dx && n)
+        HIPC(hipMemcpyAsync(dx, dx_d_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    if (phi && n)
+        HIPC(hipMemcpyAsync(phi, phi_d_, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    HIPC(hipStreamSynchronize(stream_));
 }
 
 void Engine::sync() { HIPC(hipStreamSynchronize(stream_)); }

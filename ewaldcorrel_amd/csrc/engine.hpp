@@ -65,7 +65,8 @@ public:
     // geometry / tables
     void set_cell(const double L6[6]);
     void upload_table(const double *A, const double *dA);  // H and 6H doubles
-    void build_ewald_table(double eta, double max_m);      // closed-form table
+    void build_ewald_table(double eta, double max_m);      // closed-form table (device)
+    void copy_table(double *A, double *dA);                // table to host (H, 6H)
     bool has_table() const { return has_table_; }
     int table_mode() const { return desc_.tabmode; }
 
@@ -87,6 +88,21 @@ public:
     double eval_host(int n, const double *w, const double *x, double vir[6], double *dx,
                      double *phi);
     double *get_S_host();  // host mirror of the grid, K0 x K1 x ldim x 2 doubles
+
+    // structure-factor accumulation (device arrays, FP64, caller-owned)
+    void accum_S2(double *acc_d);                          // acc[H] += |S|^2
+    void accum_cross(const Engine &b, double *acc_d);      // acc[2H] += S conj(S_b)
+    // sum_d[b] += sum of val_d over the points of |k| bin b, cnt_d[b] += count;
+    // k = 2 pi m, m from the signed principal indices, b = floor(|k| / dk)
+    void radial(const double *val_d, int nbins, double dk, double *sum_d, double *cnt_d);
+
+    // real-space Ewald sum over the images within rc plus the self and
+    // background terms, in the current box: out[10] = E_pair, vir(6), E_self,
+    // E_bg, vir_bg; dx = dE_pair/dx; phi = dE/dw of the three terms
+    void real_space(int n, const double *w_d, const double *x_d, double eta, double rc,
+                    double *out_d, double *dx_d, double *phi_d);
+    void real_space_host(int n, const double *w, const double *x, double eta, double rc,
+                         double out[10], double *dx, double *phi);
 
     void sync();
     hipStream_t stream() const { return stream_; }
@@ -158,6 +174,8 @@ private:
     FrameDesc *d_desc_ = nullptr;
     void *grid_ = nullptr;
     double *table_ = nullptr;  // A[H] then dA[6H]
+    double *table_stats_ = nullptr;  // per-workgroup table-vs-closed-form statistics
+    size_t table_stats_cap_ = 0;
     bool has_table_ = false;
     double *iB_d_[3] = {nullptr, nullptr, nullptr};
     double *result_ = nullptr;  // 8 doubles
@@ -168,6 +186,14 @@ private:
     double *pin_ = nullptr;  // pinned staging
     size_t pin_cap_ = 0;
     std::vector<double> host_grid_;
+    // |k| bins of the half-complex points for radial(): key, points by bin, offsets
+    std::vector<double> radial_key_;
+    double *real_out_ = nullptr;    // real-space results of the host-pointer form
+    double *real_parts_ = nullptr;  // real-space workgroup partials
+    size_t real_parts_cap_ = 0;
+    int *radial_perm_ = nullptr;
+    int *radial_off_ = nullptr;
+    size_t radial_off_cap_ = 0;
     FftPlan plan_[3];
     SplineArgs<float> sp_f_;
     SplineArgs<double> sp_d_;

@@ -204,9 +204,43 @@ const char *sfac_last_error(void) { return g_last_error.c_str(); }
 int sfac_set_A_ewald(void *h, double eta, double max_m) {
     return guard([&] {
         if (!(eta > 0.0)) throw std::invalid_argument("eta must be > 0");
+        if (!(max_m >= 0.0)) throw std::invalid_argument("max_m must be >= 0");
         SFac *s = as_sfac(h);
         s->engine->build_ewald_table(eta, max_m);
+        // the table now lives on the device; a host copy from an earlier
+        // callback set_A would be stale (sfac_table() copies the current one)
+        free(s->A);
+        s->A = nullptr;
+        s->dA = nullptr;
     });
+}
+
+int sfac_table(void *h, double *A, double *dA) {
+    return guard([&] { as_sfac(h)->engine->copy_table(A, dA); });
+}
+
+int sfac_accum_S2(void *h, double *acc_d) {
+    return guard([&] { as_sfac(h)->engine->accum_S2(acc_d); });
+}
+
+int sfac_accum_cross(void *ha, void *hb, double *acc_d) {
+    return guard([&] { as_sfac(ha)->engine->accum_cross(*as_sfac(hb)->engine, acc_d); });
+}
+
+int sfac_radial(void *h, const double *val_d, int nbins, double dk, double *sum_d,
+                double *cnt_d) {
+    return guard([&] { as_sfac(h)->engine->radial(val_d, nbins, dk, sum_d, cnt_d); });
+}
+
+int sfac_real_space(void *h, int n, const double *w, const double *x, double eta, double rc,
+                    double out[10], double *dx, double *phi) {
+    return guard([&] { as_sfac(h)->engine->real_space_host(n, w, x, eta, rc, out, dx, phi); });
+}
+
+int sfac_real_space_dev(void *h, int n, const double *w_d, const double *x_d, double eta,
+                        double rc, double *out_d, double *dx_d, double *phi_d) {
+    return guard(
+        [&] { as_sfac(h)->engine->real_space(n, w_d, x_d, eta, rc, out_d, dx_d, phi_d); });
 }
 
 int sfac_dev(void *h, int n, const double *w_d, const double *x_d) {

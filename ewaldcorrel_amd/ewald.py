@@ -95,6 +95,14 @@ def describe_sfac():
         decl_fn(lib.sfac_ctor_ex, nparr_t(6), np_intarr_t(3), c_int, c_int, sfac_p)
         decl_fn(lib.sfac_last_error, c_char_p)
         int_fn(lib.sfac_set_A_ewald, sfac_p, c_double, c_double)
+        int_fn(lib.sfac_table, sfac_p, c_void_p, c_void_p)
+        int_fn(lib.sfac_accum_S2, sfac_p, c_void_p)
+        int_fn(lib.sfac_accum_cross, sfac_p, sfac_p, c_void_p)
+        int_fn(lib.sfac_radial, sfac_p, c_void_p, c_int, c_double, c_void_p, c_void_p)
+        int_fn(lib.sfac_real_space, sfac_p, c_int, nparr, nparr, c_double, c_double,
+               nparr_t(10), c_void_p, c_void_p)
+        int_fn(lib.sfac_real_space_dev, sfac_p, c_int, c_void_p, c_void_p, c_double, c_double,
+               c_void_p, c_void_p, c_void_p)
         int_fn(lib.sfac_dev, sfac_p, c_int, c_void_p, c_void_p)
         int_fn(lib.en_dev, sfac_p, c_void_p)
         int_fn(lib.de1_dev, sfac_p, c_void_p)
@@ -264,6 +272,53 @@ class Sfac:
     def set_graph(self, enable=True):
         """Replay eval_dev/evaluate from a captured hipGraph (default on)."""
         _check(self._libsfac.sfac_set_graph(self._data, int(bool(enable))), "set_graph")
+
+    def table(self):
+        """(A[H], dA[H, 6]) of the current influence function, as the
+        reference's set_A builds them (sfac.cpp:213-292)."""
+        K0, K1, K2 = (int(k) for k in self.K)
+        H = K0 * K1 * (K2 // 2 + 1)
+        A = np.zeros(H)
+        dA = np.zeros((H, 6))
+        _check(self._libsfac.sfac_table(self._data, A.ctypes.data, dA.ctypes.data), "table")
+        return A, dA
+
+    # ---- structure-factor accumulation (device pointers, FP64) -----------
+    def accum_S2(self, acc_ptr):
+        """acc[H] += |S_hat|^2 after sfac (run.py:45), on the device."""
+        _check(self._libsfac.sfac_accum_S2(self._data, acc_ptr), "accum_S2")
+
+    def accum_cross(self, other, acc_ptr):
+        """acc[2H] += S_hat conj(S_hat of other) (gmx_ewald.py:101-104)."""
+        _check(self._libsfac.sfac_accum_cross(self._data, other._data, acc_ptr),
+               "accum_cross")
+
+    def radial(self, val_ptr, nbins, dk, sum_ptr, cnt_ptr):
+        """|k| shell sums and counts of a per-point array (plot.py:19-32)."""
+        _check(self._libsfac.sfac_radial(self._data, val_ptr, int(nbins), float(dk),
+                                         sum_ptr, cnt_ptr), "radial")
+
+    # ---- real-space Ewald ------------------------------------------------
+    def real_space(self, w, x, eta, rc):
+        """Real-space Ewald sum over the images within rc plus the self and
+        neutralising-background terms, in the current box.  Returns a dict
+        with E (pairs), vir (6), dx (= dE/dx of the pairs), phi (dE/dw of
+        all terms), E_self, E_bg and vir_bg."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = len(w)
+        out = np.zeros(10)
+        dx = np.zeros((n, 3))
+        phi = np.zeros(n)
+        _check(self._libsfac.sfac_real_space(self._data, n, w, x, float(eta), float(rc), out,
+                                             dx.ctypes.data, phi.ctypes.data), "real_space")
+        return dict(E=out[0], vir=out[1:7].copy(), E_self=out[7], E_bg=out[8], vir_bg=out[9],
+                    dx=dx, phi=phi)
+
+    def real_space_dev(self, n, w_ptr, x_ptr, eta, rc, out_ptr, dx_ptr=None, phi_ptr=None):
+        _check(self._libsfac.sfac_real_space_dev(self._data, n, w_ptr, x_ptr, float(eta),
+                                                 float(rc), out_ptr, dx_ptr, phi_ptr),
+               "real_space_dev")
 
     def set_transform(self, mode):
         """0: y-z slab transforms where available (default); 1: z and y line passes."""

@@ -36,6 +36,12 @@ const char *sfac_last_error(void);
  * (ewald.py:124-132), built without a per-point callback. */
 int sfac_set_A_ewald(void *h, double eta, double max_m);
 
+/* Copy of the current influence table (the reference keeps it in SFac::A,
+ * SFac::dA, sfac.hpp:184): A[H] and dA[6H], H = K0 K1 (K2/2+1); either
+ * pointer may be NULL.  After sfac_set_A_ewald the table is built on the
+ * device and SFac::A is NULL. */
+int sfac_table(void *h, double *A, double *dA);
+
 /* Device-pointer forms of sfac / en / de1 / de2 / potl (sfac.cpp:190-545).
  * en_dev writes E to out_d[0]; de1_dev writes E to out_d[0] and the virial
  * (xx,yy,zz,yx,zx,zy) to out_d[1..6]. */
@@ -87,6 +93,41 @@ int sfac_batch_eval(void *b, void *stream);
 int sfac_batch_sync(void *b);
 int sfac_batch_set_graph(void *b, int enable);
 int sfac_batch_destroy(void *b);
+
+/* Structure-factor accumulation on the device, asynchronous on the
+ * handle's stream (the callers of get_S: run.py:36-47 sums |S|^2 over
+ * frames, gmx_ewald.py:78-107 sums S_i conj(S_j) over species pairs,
+ * plot.py:9-32 averages over |k| shells).  Arrays are FP64 device memory
+ * owned by the caller, indexed like the half-complex grid (H points).
+ *   sfac_accum_S2:    after sfac, acc_d[h] += |S_hat_h|^2
+ *   sfac_accum_cross: after sfac on both handles (same grid and precision),
+ *                     acc_d[2h], acc_d[2h+1] += Re, Im of S_a conj(S_b)
+ *   sfac_radial:      sum_d[b] += sum of val_d[h] over the points with
+ *                     b = floor(|k| / dk) < nbins, cnt_d[b] += their number;
+ *                     k = 2 pi m, m from the signed indices (K/2 positive)
+ *                     and kz in [0, ldim), in the handle's current box */
+int sfac_accum_S2(void *h, double *acc_d);
+int sfac_accum_cross(void *ha, void *hb, double *acc_d);
+int sfac_radial(void *h, const double *val_d, int nbins, double dk, double *sum_d,
+                double *cnt_d);
+
+/* Real-space Ewald part and corrections (notes.tex:66-79, test.py:83,
+ * 126-132), which with en/de1 for f(z) = exp(-pi^2 z/eta^2)/(pi z) give the
+ * total Coulomb energy of the periodic system, independent of eta:
+ *   out[0]    E_pair = 1/2 sum_i sum_{j,n}' w_i w_j erfc(eta r)/r over every
+ *             periodic image with r < rc
+ *   out[1..6] its virial (xx,yy,zz,yx,zx,zy) in de1's convention
+ *   out[7]    E_self = -eta/sqrt(pi) sum w^2
+ *   out[8]    E_bg = -pi (sum w)^2 / (2 V eta^2) (neutralising background)
+ *   out[9]    E_bg / V, the background's virial on each diagonal component
+ *   dx        dE_pair/dx (NULL: not computed);  phi  dE/dw of all three terms
+ * FP64 pair sum in the handle's current box, O(n^2) (for checks and moderate
+ * n).  sfac_real_space takes host pointers and returns when done; the _dev
+ * form takes device pointers and is asynchronous on the handle's stream. */
+int sfac_real_space(void *h, int n, const double *w, const double *x, double eta, double rc,
+                    double out[10], double *dx, double *phi);
+int sfac_real_space_dev(void *h, int n, const double *w_d, const double *x_d, double eta,
+                        double rc, double *out_d, double *dx_d, double *phi_d);
 
 /* Per-kernel timing: while enabled, every launch on the handle's stream is
  * bracketed by HIP events.  sfac_profile_read synchronises, writes up to

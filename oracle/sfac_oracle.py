@@ -423,3 +423,61 @@ class OracleSfac:
         dx = self.de2(w, x)
         phi = self.potl(x)
         return E, vir, dx, phi
+
+
+# ---- real-space Ewald and corrections (notes.tex:73-79, test.py:83,126-132) --------------
+def real_space(L6, w, x, eta, rc):
+    """Real-space Ewald sum over every periodic image within the cutoff rc,
+    with the self and neutralising-background corrections, by brute force
+    (small systems; the checker of the device real-space kernel).
+
+    The reciprocal part with f(z) = exp(-pi^2 z/eta^2)/(pi z) is completed
+    by (CGS units, notes.tex:66-79)
+      E_pair = 1/2 sum_i sum_{j, n}' w_i w_j erfc(eta r)/r,  r = |x_i - x_j + L^T n|
+      E_self = -eta/sqrt(pi) sum_i w_i^2                    (test.py:83)
+      E_bg   = -pi (sum_i w_i)^2 / (2 V eta^2)
+    Returns dict(E=E_pair, vir (xx,yy,zz,yx,zx,zy) of the pairs with the
+    convention of de1 (-(1/V) sum U'(r)/r r_a r_b), dx = dE_pair/dx,
+    phi = dE/dw_i of all three terms, E_self, E_bg, vir_bg = E_bg / V on the
+    diagonal)."""
+    from math import erfc, exp, sqrt
+    cell = Cell(L6)
+    Lm = LofS(np.asarray(L6, dtype=float))
+    iL = cell.iL
+    V = 1.0 / cell.iV
+    n = len(w)
+    x = np.asarray(x, dtype=float)
+    w = np.asarray(w, dtype=float)
+    # image range: |s_d + n_d| <= rc |iL[:, d]| for every image within rc
+    ext = [int(np.ceil(rc * np.linalg.norm(iL[:, d]))) + 1 for d in range(3)]
+    E = 0.0
+    vir = np.zeros((3, 3))
+    dx = np.zeros((n, 3))
+    phi = np.zeros(n)
+    c2 = 2.0 * eta / sqrt(pi)
+    for i in range(n):
+        d = x[i] - x
+        s = d @ iL
+        s -= np.round(s)
+        for a in range(-ext[0], ext[0] + 1):
+            for b in range(-ext[1], ext[1] + 1):
+                for c in range(-ext[2], ext[2] + 1):
+                    r = (s + np.array([a, b, c])) @ Lm
+                    r2 = np.einsum("ij,ij->i", r, r)
+                    for j in np.nonzero((r2 < rc * rc) & (r2 > 1e-20))[0]:
+                        rr = sqrt(r2[j])
+                        g = erfc(eta * rr) / rr
+                        phi[i] += w[j] * g
+                        u = w[i] * w[j] * g
+                        up = w[i] * w[j] * (-g / rr - c2 * exp(-eta * eta * r2[j]) / rr)
+                        E += 0.5 * u
+                        dx[i] += up / rr * r[j]
+                        vir += 0.5 * (up / rr) * np.outer(r[j], r[j])
+    vir = -vir / V
+    Q = w.sum()
+    E_self = -eta / sqrt(pi) * np.sum(w * w)
+    E_bg = -pi * Q * Q / (2.0 * V * eta * eta)
+    phi += -c2 * w - pi * Q / (V * eta * eta)
+    return dict(E=E, vir=np.array([vir[0, 0], vir[1, 1], vir[2, 2], vir[1, 0], vir[2, 0],
+                                   vir[2, 1]]),
+                dx=dx, phi=phi, E_self=E_self, E_bg=E_bg, vir_bg=E_bg / V)

@@ -1,0 +1,122 @@
+"""Structure-factor accumulation on the device (the reference's callers of
+get_S: run.py sums |S|^2 over frames, gmx_ewald.py sums species cross terms
+S_i conj(S_j), plot.py averages over |k| shells).  Checked against the same
+sums formed on the host from the oracle's S-hat."""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle.sfac_oracle import Cell, LofS, OracleSfac  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ew():
+    from ewaldcorrel_amd import ewald
+    return ewald
+
+
+L6 = np.array([20., 22., 24., 1., -2., 1.5])
+K = (24, 25, 30)
+H = K[0] * K[1] * (K[2] // 2 + 1)
+
+
+def frame(seed, n=400):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (n, 3)) @ LofS(L6)
+    w = rng.normal(size=n)
+    return w, x
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_accum_S2_over_frames(precision, tol):
+    s = ew().Sfac(L6, np.array(K), 4, precision=precision)
+    o = OracleSfac(L6, K, 4)
+    acc = torch.zeros(H, dtype=torch.float64, device="cuda")
+    ref = np.zeros(H)
+    for seed in range(3):
+        w, x = frame(seed)
+        s.sfac(w, x)
+        s.accum_S2(acc.data_ptr())
+        o.sfac(w, x)
+        ref += np.abs(o.get_S().reshape(-1)) ** 2
+    s.sync()
+    got = acc.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say(precision, "accum |S|^2 rel err", err)
+    assert err < tol
+
+
+def test_accum_cross_species():
+    """gmx_ewald.py:99-104: C += Q_i conj(Q_j) for two species."""
+    a = ew().Sfac(L6, np.array(K), 4)
+    b = ew().Sfac(L6, np.array(K), 4)
+    oa, ob = OracleSfac(L6, K, 4), OracleSfac(L6, K, 4)
+    acc = torch.zeros(2 * H, dtype=torch.float64, device="cuda")
+    ref = np.zeros(H, dtype=complex)
+    for seed in range(2):
+        wa, xa = frame(10 + seed, 300)
+        wb, xb = frame(20 + seed, 200)
+        a.sfac(wa, xa)
+        b.sfac(wb, xb)
+        a.accum_cross(b, acc.data_ptr())
+        oa.sfac(wa, xa)
+        ob.sfac(wb, xb)
+        ref += oa.get_S().reshape(-1) * np.conj(ob.get_S().reshape(-1))
+    a.sync()
+    got = acc.cpu().numpy().reshape(H, 2)
+    got = got[:, 0] + 1j * got[:, 1]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say("cross rel err", err)
+    assert err < 1e-12
+
+
+def test_radial_average():
+    """plot.py: shell sums by |k| = 2 pi |m| with dk bins (the stored
+    half-complex points once each; K/2 taken as the positive index)."""
+    s = ew().Sfac(L6, np.array(K), 4)
+    rng = np.random.default_rng(5)
+    val = rng.uniform(0, 1, H)
+    nb, dk = 40, 0.05
+    vd = torch.tensor(val, device="cuda")
+    sm = torch.zeros(nb, dtype=torch.float64, device="cuda")
+    ct = torch.zeros(nb, dtype=torch.float64, device="cuda")
+    s.radial(vd.data_ptr(), nb, dk, sm.data_ptr(), ct.data_ptr())
+    s.radial(vd.data_ptr(), nb, dk, sm.data_ptr(), ct.data_ptr())  # accumulates
+    s.sync()
+    iL = Cell(L6).iL
+    K0, K1, K2 = K
+    nx = np.arange(K0)
+    nx = np.where(nx <= K0 // 2, nx, nx - K0)
+    ny = np.arange(K1)
+    ny = np.where(ny <= K1 // 2, ny, ny - K1)
+    nz = np.arange(K2 // 2 + 1)
+    NX, NY, NZ = np.meshgrid(nx, ny, nz, indexing="ij")
+    mx = NX * iL[0, 0]
+    my = NX * iL[1, 0] + NY * iL[1, 1]
+    mz = NX * iL[2, 0] + NY * iL[2, 1] + NZ * iL[2, 2]
+    k = 2 * np.pi * np.sqrt(mx ** 2 + my ** 2 + mz ** 2).reshape(-1)
+    b = np.floor(k / dk).astype(int)
+    keep = b < nb
+    rs = 2 * np.bincount(b[keep], weights=val[keep], minlength=nb)
+    rc = 2 * np.bincount(b[keep], minlength=nb)
+    gs, gc = sm.cpu().numpy(), ct.cpu().numpy()
+    say("radial", np.abs(gs - rs).max(), np.abs(gc - rc).max())
+    assert np.array_equal(gc, rc)
+    assert np.abs(gs - rs).max() <= 1e-12 * np.abs(rs).max()
+
+
+def test_accumulation_requires_sfac(capfd):
+    s = ew().Sfac(L6, np.array(K), 4)
+    acc = torch.zeros(H, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError):
+        s.accum_S2(acc.data_ptr())
+    assert "You must call sfac" in capfd.readouterr().out
