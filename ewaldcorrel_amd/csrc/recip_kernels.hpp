@@ -28,8 +28,11 @@ namespace spme {
 // 1e-10 <= |m|^2 <= max_m^2; f(z) = exp(fac z)/(pi z), f' = (fac - 1/z) f
 // (ewald.py:124-132).  The engine only selects this mode after checking
 // on the host that it reproduces the full table.
+// Contraction off: the device table builder evaluates the same expressions
+// for the same image, so the two agree bitwise where no alias contributes.
 __host__ __device__ inline void ewald_image(const CellArgs &c, double fac, double max_m2,
                                             int ix, int iy, int iz, double &a, double *d) {
+#pragma clang fp contract(off)
     const double nx = ix, ny = iy, nz = iz;
     const double mx = nx * c.iL00;
     const double my = nx * c.iL10 + ny * c.iL11;
