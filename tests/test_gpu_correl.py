@@ -120,3 +120,31 @@ def test_accumulation_requires_sfac(capfd):
     with pytest.raises(RuntimeError):
         s.accum_S2(acc.data_ptr())
     assert "You must call sfac" in capfd.readouterr().out
+
+
+def test_accumulate_sk_over_xyz_frames(tmp_path):
+    """run.py: |S|^2 of neutron-weighted frames read from .xyz, summed on
+    the device, equals the host sum over the oracle's S-hat."""
+    from ewaldcorrel_amd.traj import accumulate_sk, neutron_weights, read_xyz
+    rng = np.random.default_rng(9)
+    names = ["O", "H", "H", "C"] * 50
+    text = ""
+    frames = []
+    for f in range(3):
+        x = rng.uniform(0, 20, (len(names), 3))
+        frames.append(x)
+        text += f"{len(names)}\nframe {f}\n" + "".join(
+            f"{nm} {a:.10f} {b:.10f} {c:.10f}\n" for nm, (a, b, c) in zip(names, x))
+    p = tmp_path / "t.xyz"
+    p.write_text(text)
+    L6c = np.array([20., 20., 20., 0., 0., 0.])
+    s = ew().Sfac(L6c, np.array([24, 24, 24]), 4)
+    acc, nfr = accumulate_sk(s, read_xyz(p))
+    o = OracleSfac(L6c, (24, 24, 24), 4)
+    ref = 0
+    for nm_x in read_xyz(p):
+        o.sfac(neutron_weights(nm_x[0]), nm_x[1])
+        ref = ref + np.abs(o.get_S()) ** 2
+    err = np.abs(acc - ref).max() / np.abs(ref).max()
+    say("xyz frames", nfr, "rel err", err)
+    assert nfr == 3 and err < 1e-12
