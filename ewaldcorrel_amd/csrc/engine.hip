@@ -129,16 +129,20 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_XF_SCALE: fft_x_kernel<T, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
         case ST_XFUSED: fft_x_kernel<T, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
         case ST_YZ_FWD:
-            if (L.slab_ept == kSlabEptSmall)
-                fft_yz_fwd_kernel<T, kSlabEptSmall><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
+            if (L.slab_ept == SlabCfg<T>::small)
+                fft_yz_fwd_kernel<T, SlabCfg<T>::small>
+                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf);
             else
-                fft_yz_fwd_kernel<T, slab_ept<T>()><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf);
+                fft_yz_fwd_kernel<T, SlabCfg<T>::large>
+                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf);
             break;
         case ST_YZ_INV:
-            if (L.slab_ept == kSlabEptSmall)
-                fft_yz_inv_kernel<T, kSlabEptSmall><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
+            if (L.slab_ept == SlabCfg<T>::small)
+                fft_yz_inv_kernel<T, SlabCfg<T>::small>
+                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf, finish);
             else
-                fft_yz_inv_kernel<T, slab_ept<T>()><<<g, kSlabBlock, lds, st>>>(L.frames, L.offs, L.nf, finish);
+                fft_yz_inv_kernel<T, SlabCfg<T>::large>
+                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf, finish);
             break;
         case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
@@ -200,10 +204,10 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        const void *fns[4] = {(const void *)fft_yz_fwd_kernel<T, kSlabEptSmall>,
-                              (const void *)fft_yz_fwd_kernel<T, slab_ept<T>()>,
-                              (const void *)fft_yz_inv_kernel<T, kSlabEptSmall>,
-                              (const void *)fft_yz_inv_kernel<T, slab_ept<T>()>};
+        const void *fns[4] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
+                              (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::large>,
+                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::small>,
+                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::large>};
         ok = true;
         for (const void *fn : fns)
             ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -341,6 +345,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
         const size_t lds = fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
                                 : slab_lds_bytes<double>(K[1], K[2], ldim);
         slab_ok = !D.plan1.generic && !D.plan2.generic && lds <= kSlabLds &&
+                  slab_ept() > 0 &&
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
     }
@@ -592,7 +597,7 @@ void Engine::launch(Stage s, const Launch &L, int blocks, size_t lds,
 }
 
 int Engine::slab_ept() const {
-    return slab_ept_for(K[1], ldim, fp32 ? spme::slab_ept<float>() : spme::slab_ept<double>());
+    return fp32 ? slab_ept_for<float>(K[1], K[2], ldim) : slab_ept_for<double>(K[1], K[2], ldim);
 }
 
 void Engine::sync_desc_() {

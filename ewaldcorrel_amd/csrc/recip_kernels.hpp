@@ -355,11 +355,10 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
     double *ib0 = reinterpret_cast<double *>(twl + N);
     cplx<T> *lines = reinterpret_cast<cplx<T> *>(ib0 + ((N + 1) & ~1));
     cplx<T> *lines2 = lines + C * LS;  // generic plans only
+    // HBM <-> LDS mapping: column c = t mod C, elements t div C + q (256 / C)
     const int c = threadIdx.x & (C - 1);
     const int e0 = threadIdx.x >> lg, es = kBlock >> lg;
-    const int f = b * C + c;          // this thread's column
-    const bool valid = f < P;
-    const int fc = valid ? f : P - 1;  // clamped for loads
+    const int fc = min(b * C + c, P - 1);  // clamped for loads
     cplx<T> *col = (cplx<T> *)F.grid + fc;
     {
         T vx[EPT], vy[EPT];
@@ -385,15 +384,40 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         }
     }
     __syncthreads();
-    FastDiv dC;
-    dC.d = C;
-    dC.m = 1;  // power of two: n div C = n >> lg
-    dC.s = lg;
+    // Transforms: radices 2..8 and C >= 4 -- every wavefront owns C/4 whole
+    // lines and runs its stages alone; otherwise the workgroup's transform.  The influence step runs on the lanes that own the line:
+    // line l0 + (lane mod G), elements lane div G + q (64 / G) (G a power of
+    // two); with the ping-pong transform, the HBM mapping above.
     const int s0 = MODE == 0 ? sign : -1;
-    cplx<T> *res = x_transform<T>(lines, lines2, C, dC, LS, p, twl, s0);
-    const int ky = fdiv(fc, F.dldim), kz = fc - ky * ldim;
-    const double ibyz = (MODE != 0) ? F.iB1[ky] * F.iB2[kz] : 1.0;
+    const bool wave = !p.generic && C >= 4;  // a wavefront's share fits EPT per lane
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int l0 = 0, G = 0;
+    wave_share(C, kBlock / 64, wv, l0, G);
+    cplx<T> *res = lines;
+    if (wave) {
+        if (G > 0) wave_lines_fft<T, EPT, true>(lines, l0, G, LS, 1, p, twl, s0);
+    } else {
+        FastDiv dC;
+        dC.d = C;
+        dC.m = 1;  // power of two: n div C = n >> lg
+        dC.s = lg;
+        res = x_transform<T>(lines, lines2, C, dC, LS, p, twl, s0);
+    }
     if (MODE == 2) {
+        int cl = c, ie0 = e0, ies = es;
+        bool own = true;
+        if (wave) {
+            own = G > 0;
+            const int lgG = G > 1 ? 31 - __clz(G) : 0;
+            cl = l0 + (lane & (G - 1));
+            ie0 = lane >> lgG;
+            ies = 64 >> lgG;
+        }
+        const int f = b * C + cl;
+        const bool valid = own && f < P;
+        const int fv = valid ? f : P - 1;
+        const int ky = fdiv(fv, F.dldim), kz = fv - ky * ldim;
+        const double ibyz = F.iB1[ky] * F.iB2[kz];
         const bool even = (F.K2 % 2) == 0;
         const double two = (kz == 0 || (even && kz == ldim - 1)) ? 2.0 : 1.0;
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -402,10 +426,10 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
             const float ibyzf = (float)ibyz, twof = (float)two;
 #pragma unroll 1
             for (int q = 0; q < EPT; ++q) {
-                const int e = e0 + q * es;
+                const int e = ie0 + q * ies;
                 if (e < N && valid) {
                     const float ib = (float)ib0[e] * ibyzf;
-                    cplx<T> v = res[c * LS + e];
+                    cplx<T> v = res[cl * LS + e];
                     const float re = (float)v.x * ib, im = (float)v.y * ib;  // S-hat
                     const float s = re * re + im * im;
                     float A, dA[6];
@@ -416,16 +440,16 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
                     const float tt = A * ib * twof;
                     v.x = T(re * tt);
                     v.y = T(im * tt);
-                    res[c * LS + e] = v;
+                    res[cl * LS + e] = v;
                 }
             }
         } else {
 #pragma unroll 1
             for (int q = 0; q < EPT; ++q) {
-                const int e = e0 + q * es;
+                const int e = ie0 + q * ies;
                 if (e < N && valid) {
                     const double ib = ib0[e] * ibyz;
-                    cplx<T> v = res[c * LS + e];
+                    cplx<T> v = res[cl * LS + e];
                     const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
                     const double s = re * re + im * im;
                     double A, dA[6];
@@ -436,24 +460,36 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
                     const double tt = A * ib * two;
                     v.x = T(re * tt);
                     v.y = T(im * tt);
-                    res[c * LS + e] = v;
+                    res[cl * LS + e] = v;
                 }
             }
         }
+        if (wave) {
+            wave_lds_sync();
+            if (G > 0) wave_lines_fft<T, EPT, true>(lines, l0, G, LS, 1, p, twl, +1);
+        } else {
+            __syncthreads();
+            res = x_transform<T>(res, res == lines ? lines2 : lines, C, FastDiv{(unsigned)C, 1u,
+                                 (unsigned)lg}, LS, p, twl, +1);
+        }
         __syncthreads();
-        res = x_transform<T>(res, res == lines ? lines2 : lines, C, dC, LS, p, twl, +1);
+        const bool vs = b * C + c < P;
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = e0 + q * es;
-            if (e < N && valid) col[(long)e * P] = res[c * LS + e];
+            if (e < N && vs) col[(long)e * P] = res[c * LS + e];
         }
         block_partial(F, b, acc);
         return;
     }
+    __syncthreads();
+    const bool vs = b * C + c < P;
+    const int ky = fdiv(fc, F.dldim), kz = fc - ky * ldim;
+    const double ibyz = (MODE == 1) ? F.iB1[ky] * F.iB2[kz] : 1.0;
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = e0 + q * es;
-        if (e < N && valid) {
+        if (e < N && vs) {
             cplx<T> v = res[c * LS + e];
             if (MODE == 1) {
                 const T s = T(ib0[e] * ibyz);

@@ -23,27 +23,34 @@
 
 namespace spme {
 
-constexpr int kSlabBlock = 1024;
+// FP32 planes (<= 80 KB: two workgroups per CU) use 512 threads held to
+// 128 VGPRs so that both fit; FP64 planes (one workgroup per CU by LDS) use
+// 1024 threads.
+template <typename T> struct SlabCfg;
+template <> struct SlabCfg<float> {
+    static constexpr int block = 512, small = 20, large = 32;
+};
+template <> struct SlabCfg<double> {
+    static constexpr int block = 1024, small = 10, large = 20;
+};
 // dynamic LDS budget: gfx950 has 160 KB per workgroup, 1 KB of which is
 // left to the static arrays of the reductions
 constexpr size_t kSlabLds = 159 * 1024;
 
-// largest slab (complex elements) a workgroup can hold, and elements per thread
-template <typename T> __host__ __device__ constexpr int slab_elems() {
-    return (int)(kSlabLds / sizeof(cplx<T>));
-}
-template <typename T> __host__ __device__ constexpr int slab_ept() {
-    return (slab_elems<T>() + kSlabBlock - 1) / kSlabBlock;
-}
 
 __host__ __device__ inline int slab_rows(int K1) { return (K1 + 1) & ~1; }
 
-// planes of up to 9 x 1024 elements (128 x 65 among them) use a variant
-// compiled for 9 elements per thread: fewer registers held per stage
-constexpr int kSlabEptSmall = 9;
-__host__ __device__ inline int slab_ept_for(int K1, int ldim, int ept_max) {
-    return (size_t)slab_rows(K1) * ldim <= (size_t)kSlabEptSmall * kSlabBlock ? kSlabEptSmall
-                                                                            : ept_max;
+// The z lines (row pairs) and the y lines (columns) are dealt to the
+// workgroup's wavefronts; a wavefront's share of elements per lane decides
+// the variant (small or large, compiled separately; 0: the slab does not
+// apply).
+template <typename T> __host__ __device__ inline int slab_ept_for(int K1, int K2, int ldim) {
+    constexpr int W = SlabCfg<T>::block / 64;
+    const int npair = (K1 + 1) / 2;
+    const int gz = (npair + W - 1) / W, gy = (ldim + W - 1) / W;
+    const int need = (max(gz * K2, gy * K1) + 63) / 64;
+    return need <= SlabCfg<T>::small ? SlabCfg<T>::small
+                                     : need <= SlabCfg<T>::large ? SlabCfg<T>::large : 0;
 }
 
 template <typename T>
@@ -54,13 +61,16 @@ __host__ __device__ inline size_t slab_lds_bytes(int K1, int K2, int ldim) {
 template <typename T>
 __device__ __forceinline__ void slab_twiddles(cplx<T> *twz, cplx<T> *twy, const FrameDesc &F) {
     const cplx<T> *gz = (const cplx<T> *)F.tw2, *gy = (const cplx<T> *)F.tw1;
-    for (int t = threadIdx.x; t < F.K2; t += kSlabBlock) twz[t] = gz[t];
-    for (int t = threadIdx.x; t < F.K1; t += kSlabBlock) twy[t] = gy[t];
+    for (int t = threadIdx.x; t < F.K2; t += (int)blockDim.x) twz[t] = gz[t];
+    for (int t = threadIdx.x; t < F.K1; t += (int)blockDim.x) twy[t] = gy[t];
 }
 
 template <typename T, int EPT>
-__global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS) {
+__global__ void __launch_bounds__(SlabCfg<T>::block)
+    __attribute__((amdgpu_waves_per_eu(4)))
+    fft_yz_fwd_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int kSlabBlock = SlabCfg<T>::block, kSlabWaves = kSlabBlock / 64;
     constexpr int U = 8;  // staging registers per round
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -95,25 +105,28 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS)
         }
     }
     __syncthreads();
-    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, -1);
-    // untangle the pairs into rows 2c and 2c+1 (all reads before the barrier)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int l0, G;
+    wave_share(npair, kSlabWaves, wv, l0, G);
+    wave_lines_fft<T, EPT, false>(slab, l0, G, 2 * RS, 1, F.plan2, twz, -1);
+    // untangle the wavefront's pairs into rows 2c and 2c+1 (reads first)
     {
-        constexpr int Q = (EPT + 1) / 2;
-        const int nk = npair * RS;
+        constexpr int Q = EPT / 2 + 1;
+        const int nk = G * RS;
         cplx<T> za[Q], zb[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = min((int)threadIdx.x + q * kSlabBlock, nk - 1);
-            const int c = fdiv(l, F.dldim), k = l - c * RS;
+            const int l = max(min(lane + q * 64, nk - 1), 0);
+            const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
             za[q] = slab[c * 2 * RS + k];
             zb[q] = slab[c * 2 * RS + (k == 0 ? 0 : K2 - k)];
         }
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = (int)threadIdx.x + q * kSlabBlock;
+            const int l = lane + q * 64;
             if (l < nk) {
-                const int c = fdiv(l, F.dldim), k = l - c * RS;
+                const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
                 const cplx<T> zk = za[q], zm = zb[q];
                 cplx<T> a, bb;
                 a.x = T(0.5) * (zk.x + zm.x);
@@ -127,7 +140,9 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS)
         __syncthreads();
     }
     // y transform: ldim lines (columns kz), elements at stride RS
-    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, -1);
+    wave_share(RS, kSlabWaves, wv, l0, G);
+    wave_lines_fft<T, EPT, true>(slab, l0, G, 1, RS, F.plan1, twy, -1);
+    __syncthreads();
     cplx<T> *out = (cplx<T> *)F.grid + (long)x * K1 * RS;
     for (int l = threadIdx.x; l < K1 * RS; l += kSlabBlock) out[l] = slab[l];
 }
@@ -135,8 +150,11 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_fwd_kernel(SPME_FRAME_ARGS)
 // finish 1: workgroup 0 of each frame first finalizes the reduction of the
 // fused x pass that ran before.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(kSlabBlock) fft_yz_inv_kernel(SPME_FRAME_ARGS, int finish) {
+__global__ void __launch_bounds__(SlabCfg<T>::block)
+    __attribute__((amdgpu_waves_per_eu(4)))
+    fft_yz_inv_kernel(SPME_FRAME_ARGS, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int kSlabBlock = SlabCfg<T>::block, kSlabWaves = kSlabBlock / 64;
     constexpr int U = 8;
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -179,26 +197,32 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_inv_kernel(SPME_FRAME_ARGS,
         }
     }
     __syncthreads();
-    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
-    // Z_k = A_k + i B_k and the Hermitian completion (all reads before the barrier)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int l0, G;
+    wave_share(RS, kSlabWaves, wv, l0, G);
+    wave_lines_fft<T, EPT, true>(slab, l0, G, 1, RS, F.plan1, twy, +1);
+    __syncthreads();
+    // Z_k = A_k + i B_k and the Hermitian completion of the wavefront's
+    // pairs (reads first)
+    wave_share(npair, kSlabWaves, wv, l0, G);
     {
-        constexpr int Q = (EPT + 1) / 2;
-        const int nk = npair * RS;
+        constexpr int Q = EPT / 2 + 1;
+        const int nk = G * RS;
         const bool even = (K2 % 2) == 0;
         cplx<T> ra[Q], rb[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = min((int)threadIdx.x + q * kSlabBlock, nk - 1);
-            const int c = fdiv(l, F.dldim), k = l - c * RS;
+            const int l = max(min(lane + q * 64, nk - 1), 0);
+            const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
             ra[q] = slab[(2 * c) * RS + k];
             rb[q] = slab[(2 * c + 1) * RS + k];
         }
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = (int)threadIdx.x + q * kSlabBlock;
+            const int l = lane + q * 64;
             if (l < nk) {
-                const int c = fdiv(l, F.dldim), k = l - c * RS;
+                const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
                 cplx<T> a = ra[q], bb = rb[q];
                 if (k == 0 || (even && k == K2 / 2)) {
                     a.y = T(0);
@@ -217,9 +241,10 @@ __global__ void __launch_bounds__(kSlabBlock) fft_yz_inv_kernel(SPME_FRAME_ARGS,
                 }
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
-    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, +1);
+    wave_lines_fft<T, EPT, false>(slab, l0, G, 2 * RS, 1, F.plan2, twz, +1);
+    __syncthreads();
     T *g = (T *)F.grid + (long)x * K1 * 2 * RS;
     const int nz = npair * K2;
     for (int l = threadIdx.x; l < nz; l += kSlabBlock) {
