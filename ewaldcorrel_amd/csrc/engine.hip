@@ -106,6 +106,22 @@ template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, in
     return s;
 }
 
+template <typename T, int MODE>
+void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int sign,
+              int finish) {
+    switch (L.xpath) {
+        case kXWave:
+            fft_x_kernel<T, MODE, kXWave><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+            break;
+        case kXBlock:
+            fft_x_kernel<T, MODE, kXBlock><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+            break;
+        default:
+            fft_x_kernel<T, MODE, kXGeneric><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+            break;
+    }
+}
+
 template <typename T, int ORDER>
 void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
               hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
@@ -126,8 +142,12 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         }
         case ST_ZR2C: fft_z_r2c_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_YF: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
-        case ST_XF_SCALE: fft_x_kernel<T, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
-        case ST_XFUSED: fft_x_kernel<T, 2><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
+        case ST_XF_SCALE:
+            launch_x<T, 1>(L, g, bl, lds, st, -1, 0);
+            break;
+        case ST_XFUSED:
+            launch_x<T, 2>(L, g, bl, lds, st, -1, 0);
+            break;
         case ST_YZ_FWD:
             if (L.slab_ept == SlabCfg<T>::small)
                 fft_yz_fwd_kernel<T, SlabCfg<T>::small>
@@ -146,7 +166,9 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             break;
         case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
-        case ST_XI: fft_x_kernel<T, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, +1, finish); break;
+        case ST_XI:
+            launch_x<T, 0>(L, g, bl, lds, st, +1, finish);
+            break;
         case ST_YI: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, +1, finish); break;
         case ST_FINISH: finish_kernel<<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_ZC2R: fft_z_c2r_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
@@ -325,10 +347,9 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     }
     {
         // x lines: the largest power of two C <= 64 of (ky, kz) columns with
-        // C K0 elements per workgroup (fewer for the ping-pong transform)
+        // C K0 <= kXElems elements per workgroup
         const int P = K[1] * ldim;
-        const int cap = D.plan0.generic ? (fp32 ? fft_elems<float>() : fft_elems<double>())
-                                        : kXElems;
+        const int cap = kXElems;
         int lg = 0;
         while ((2 << lg) <= 64 && (2 << lg) * K[0] <= cap && (1 << lg) < P) ++lg;
         if ((1 << lg) * K[0] > cap)
@@ -612,7 +633,8 @@ void Engine::run_(Stage s, int finish) {
     if (nb <= 0) return;
     sync_desc_();
     ProfScope ps(this, stage_name(s));
-    launch(s, Launch{d_desc_, nullptr, 1, slab_ept()}, nb, stage_lds(s), stream_, finish);
+    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_)}, nb, stage_lds(s), stream_,
+           finish);
 }
 
 void Engine::run_seq_(StageSeq seq) {
@@ -994,6 +1016,7 @@ Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
     for (const Engine *e : frames_) {
         slab_ = slab_ && e->slab();
         slab_ept_ = std::max(slab_ept_, e->slab_ept());
+        xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
     }
     HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
     HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
@@ -1059,7 +1082,7 @@ void Batch::evaluate(hipStream_t st) {
         const StageSeq seq = eval_stages(slab_);
         for (int i = 0; i < seq.n; ++i) {
             const Stage s = seq.s[i];
-            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_};
+            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
             e0->prof_end_on(st);

@@ -46,6 +46,7 @@ struct Launch {
     const int *offs;
     int nf;
     int slab_ept;  // elements per thread the slab kernels are compiled for
+    int xpath;     // transform path of the x launches (XPath)
 };
 
 struct FftPlan {
@@ -243,6 +244,7 @@ private:
     bool dirty_ = true;
     bool slab_ = false;                     // every frame runs the slab transforms
     int slab_ept_ = 0;                      // slab kernel variant (largest frame)
+    int xpath_ = 0;                         // x transform path valid for every frame
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

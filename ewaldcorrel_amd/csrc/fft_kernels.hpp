@@ -152,10 +152,10 @@ __device__ __forceinline__ void bfly8(cplx<T> *v, int sign) {
 // In-place stage of fixed radix R over C lines (line c at buf + c*LS):
 // every thread loads the inputs of up to MAXQ butterflies into registers,
 // transforms them, and after a barrier stores the outputs.
-template <typename T, int R>
+template <typename T, int R, int EPT = fft_elems<T>() / kBlock>
 __device__ __forceinline__ void stage_ip(cplx<T> *buf, int C, int LS, int N, int Ns,
                                          const cplx<T> *tw, int sign) {
-    constexpr int MAXQ = (fft_elems<T>() / kBlock + R - 1) / R;
+    constexpr int MAXQ = (EPT + R - 1) / R;
     const int NR = N / R;
     const int nb = C * NR;
     const int twstep = N / (Ns * R);
@@ -226,7 +226,8 @@ __device__ __forceinline__ void stage_generic(const cplx<T> *__restrict__ src,
 }
 
 // All stages of the plan; returns the buffer that holds the result.
-template <typename T>
+// Requires C N <= EPT kBlock.
+template <typename T, int EPT = fft_elems<T>() / kBlock>
 __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, int LS,
                                               const FftPlanArgs &p, const cplx<T> *tw,
                                               int sign) {
@@ -235,11 +236,11 @@ __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, i
     for (int s = 0; s < p.nstages; ++s) {
         const int R = p.radix[s];
         switch (R) {
-            case 2: stage_ip<T, 2>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 3: stage_ip<T, 3>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 4: stage_ip<T, 4>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 5: stage_ip<T, 5>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 8: stage_ip<T, 8>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 2: stage_ip<T, 2, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 3: stage_ip<T, 3, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 4: stage_ip<T, 4, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 5: stage_ip<T, 5, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 8: stage_ip<T, 8, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
             default: {
                 stage_generic<T>(cur, oth, C, LS, p.N, Ns, R, tw, sign);
                 cplx<T> *t = cur;

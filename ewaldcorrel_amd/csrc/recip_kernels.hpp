@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(kBlock) fft_y_kernel(SPME_FRAME_ARGS, int sign
 //         reciprocal-space step of an evaluation in one HBM pass.
 // LDS: twiddles (N), iB0 (N doubles, padded to even), C lines of stride N|1
 // (two sets for a plan with a radix other than 2, 3, 4, 5, 8, which runs
-// the ping-pong line transform with C N <= fft_elems<T>()).
+// the ping-pong line transform).
 constexpr int kXElems = 2048;  // complex elements per x workgroup (8 per thread)
 
 __host__ __device__ inline int xline_stride(int N) { return N | 1; }
@@ -327,17 +327,29 @@ __host__ __device__ inline size_t xlines_lds_bytes(int N, int C, int generic) {
            (size_t)((N + 1) & ~1) * sizeof(double);
 }
 
-// x transform of the C lines in LDS; returns the buffer holding the result
-template <typename T>
+// x transform of the C lines in LDS by the whole workgroup (plans with a
+// generic radix, or fewer than four lines); returns the buffer holding the
+// result
+template <typename T, bool GENERIC>
 __device__ __forceinline__ cplx<T> *x_transform(cplx<T> *buf, cplx<T> *other, int C,
                                                 const FastDiv &dC, int LS, const FftPlanArgs &p,
                                                 const cplx<T> *tw, int sign) {
-    if (p.generic) return fft_lines<T>(buf, other, C, LS, p, tw, sign);
+    if (GENERIC) return fft_lines<T, kXElems / kBlock>(buf, other, C, LS, p, tw, sign);
     lines_fft<T, kXElems / kBlock, kBlock>(buf, C, dC, LS, 1, p, tw, sign);
     return buf;
 }
 
-template <typename T, int MODE>
+// transform of the x launches: 0 wavefront-owned lines (radices 2..8, at
+// least four lines per workgroup), 1 the workgroup's lines (radices 2..8),
+// 2 the workgroup's ping-pong transform (any radix)
+enum XPath { kXWave = 0, kXBlock = 1, kXGeneric = 2 };
+__host__ __device__ inline int x_path(const FrameDesc &F) {
+    return F.plan0.generic ? kXGeneric : kXBlock;
+}
+
+// XP: XPath, a separate instantiation each so that none holds the
+// registers of another path
+template <typename T, int MODE, int XP>
 __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int EPT = kXElems / kBlock;
@@ -389,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
     // line l0 + (lane mod G), elements lane div G + q (64 / G) (G a power of
     // two); with the ping-pong transform, the HBM mapping above.
     const int s0 = MODE == 0 ? sign : -1;
-    const bool wave = !p.generic && C >= 4;  // a wavefront's share fits EPT per lane
+    constexpr bool wave = XP == kXWave;  // every frame of the launch has this x_path()
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int l0 = 0, G = 0;
     wave_share(C, kBlock / 64, wv, l0, G);
@@ -401,7 +413,7 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         dC.d = C;
         dC.m = 1;  // power of two: n div C = n >> lg
         dC.s = lg;
-        res = x_transform<T>(lines, lines2, C, dC, LS, p, twl, s0);
+        res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
     }
     if (MODE == 2) {
         int cl = c, ie0 = e0, ies = es;
@@ -469,8 +481,9 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
             if (G > 0) wave_lines_fft<T, EPT, true>(lines, l0, G, LS, 1, p, twl, +1);
         } else {
             __syncthreads();
-            res = x_transform<T>(res, res == lines ? lines2 : lines, C, FastDiv{(unsigned)C, 1u,
-                                 (unsigned)lg}, LS, p, twl, +1);
+            res = x_transform<T, XP == kXGeneric>(res, res == lines ? lines2 : lines, C,
+                                                  FastDiv{(unsigned)C, 1u, (unsigned)lg}, LS,
+                                                  p, twl, +1);
         }
         __syncthreads();
         const bool vs = b * C + c < P;
