@@ -48,13 +48,6 @@ inline FastDiv make_fastdiv(unsigned d) {
     f.m = (unsigned)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
     return f;
 }
-__device__ __forceinline__ FastDiv make_fastdiv_dev(unsigned d) {
-    FastDiv f;
-    f.d = d;
-    f.s = d > 1 ? 32 - __clz(d - 1) : 0;  // ceil(log2 d)
-    f.m = (unsigned)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
-    return f;
-}
 __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
     return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
 }

@@ -109,17 +109,10 @@ template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, in
 template <typename T, int MODE>
 void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int sign,
               int finish) {
-    switch (L.xpath) {
-        case kXWave:
-            fft_x_kernel<T, MODE, kXWave><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
-            break;
-        case kXBlock:
-            fft_x_kernel<T, MODE, kXBlock><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
-            break;
-        default:
-            fft_x_kernel<T, MODE, kXGeneric><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
-            break;
-    }
+    if (L.xpath == kXBlock)
+        fft_x_kernel<T, MODE, kXBlock><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+    else
+        fft_x_kernel<T, MODE, kXGeneric><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
 }
 
 template <typename T, int ORDER>

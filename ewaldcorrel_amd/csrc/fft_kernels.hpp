@@ -328,103 +328,6 @@ __device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC
     }
 }
 
-// ---- wavefront-owned lines ------------------------------------------------------------------
-// The same stages for the lines [l0, l0 + G) that one wavefront owns alone:
-// its 64 lanes share the butterflies, and consecutive stages need only the
-// wavefront's own ordering (LDS operations of a wavefront are performed in
-// issue order; wave_lds_sync keeps the compiler from moving them), not a
-// workgroup barrier, so wavefronts proceed independently.  LFAST: lanes
-// walk lines fastest (for lines adjacent in LDS), else elements fastest.
-// Requires G N <= EPW 64.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename T, int R, int EPW, bool LFAST>
-__device__ __forceinline__ void wstage(cplx<T> *buf, int l0, int G, const FastDiv &dG, int LST,
-                                       int EST, int N, int Ns, const FastDiv &dNs,
-                                       const FastDiv &dNR, int twstep, const cplx<T> *tw,
-                                       int sign, int lane) {
-    constexpr int MAXQ = (EPW + R - 1) / R;
-    const int NR = N / R;
-    const int nb = G * NR;
-    cplx<T> v[MAXQ][R];
-    int dst[MAXQ];
-#pragma unroll
-    for (int q = 0; q < MAXQ; ++q) {
-        const int bf = lane + q * 64;
-        dst[q] = -1;
-        if (bf < nb) {
-            int l, j;
-            if (LFAST) {
-                j = fdiv(bf, dG);
-                l = bf - j * G;
-            } else {
-                l = fdiv(bf, dNR);
-                j = bf - l * NR;
-            }
-            const int jq = fdiv(j, dNs);
-            const int k = j - jq * Ns;
-            const cplx<T> *s = buf + (l0 + l) * LST + j * EST;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = s[r * NR * EST];
-            if (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[q][r] = cmul<T>(v[q][r], twiddle<T>(tw, r * k * twstep, sign));
-            }
-            if (R == 2) bfly2<T>(v[q]);
-            else if (R == 3) bfly3<T>(v[q], sign);
-            else if (R == 4) bfly4<T>(v[q], sign);
-            else if (R == 5) bfly5<T>(v[q], sign);
-            else if (R == 8) bfly8<T>(v[q], sign);
-            dst[q] = (l0 + l) * LST + (j + jq * Ns * (R - 1)) * EST;
-        }
-    }
-    wave_lds_sync();
-#pragma unroll
-    for (int q = 0; q < MAXQ; ++q) {
-        if (dst[q] >= 0) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns * EST] = v[q][r];
-        }
-    }
-    wave_lds_sync();
-}
-
-// All stages of a plan with radices 2, 3, 4, 5, 8 on the wavefront's lines.
-template <typename T, int EPW, bool LFAST>
-__device__ __forceinline__ void wave_lines_fft(cplx<T> *buf, int l0, int G, int LST, int EST,
-                                               const FftPlanArgs &p, const cplx<T> *tw,
-                                               int sign) {
-    const int lane = threadIdx.x & 63;
-    const FastDiv dG = make_fastdiv_dev(G);
-    int Ns = 1;
-    for (int s = 0; s < p.nstages; ++s) {
-        const int R = p.radix[s];
-        const FastDiv &dNs = p.dNs[s];
-        const FastDiv &dNR = p.dNR[s];
-        const int ts = p.twstep[s];
-        switch (R) {
-            case 2: wstage<T, 2, EPW, LFAST>(buf, l0, G, dG, LST, EST, p.N, Ns, dNs, dNR, ts, tw, sign, lane); break;
-            case 3: wstage<T, 3, EPW, LFAST>(buf, l0, G, dG, LST, EST, p.N, Ns, dNs, dNR, ts, tw, sign, lane); break;
-            case 4: wstage<T, 4, EPW, LFAST>(buf, l0, G, dG, LST, EST, p.N, Ns, dNs, dNR, ts, tw, sign, lane); break;
-            case 5: wstage<T, 5, EPW, LFAST>(buf, l0, G, dG, LST, EST, p.N, Ns, dNs, dNR, ts, tw, sign, lane); break;
-            default: wstage<T, 8, EPW, LFAST>(buf, l0, G, dG, LST, EST, p.N, Ns, dNs, dNR, ts, tw, sign, lane); break;
-        }
-        Ns *= R;
-    }
-}
-
-// lines [l0, l0 + G) of wavefront w when C lines are dealt to nw wavefronts
-__device__ __forceinline__ void wave_share(int C, int nw, int w, int &l0, int &G) {
-    const int q = C / nw, r = C % nw;
-    G = q + (w < r ? 1 : 0);
-    l0 = w * q + min(w, r);
-}
-
 // Twiddles to LDS: all loads first (clamped index), then the LDS writes.
 template <typename T>
 __device__ __forceinline__ void load_twiddles(cplx<T> *twl, const void *tw, int N) {

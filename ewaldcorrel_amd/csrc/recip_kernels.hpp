@@ -339,16 +339,14 @@ __device__ __forceinline__ cplx<T> *x_transform(cplx<T> *buf, cplx<T> *other, in
     return buf;
 }
 
-// transform of the x launches: 0 wavefront-owned lines (radices 2..8, at
-// least four lines per workgroup), 1 the workgroup's lines (radices 2..8),
-// 2 the workgroup's ping-pong transform (any radix)
-enum XPath { kXWave = 0, kXBlock = 1, kXGeneric = 2 };
+// transform of the x launches: the workgroup's lines with radices 2..8, or
+// the ping-pong transform for any radix (a separate instantiation each, so
+// that neither holds the other's registers)
+enum XPath { kXBlock = 0, kXGeneric = 1 };
 __host__ __device__ inline int x_path(const FrameDesc &F) {
     return F.plan0.generic ? kXGeneric : kXBlock;
 }
 
-// XP: XPath, a separate instantiation each so that none holds the
-// registers of another path
 template <typename T, int MODE, int XP>
 __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -396,37 +394,13 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         }
     }
     __syncthreads();
-    // Transforms: radices 2..8 and C >= 4 -- every wavefront owns C/4 whole
-    // lines and runs its stages alone; otherwise the workgroup's transform.  The influence step runs on the lanes that own the line:
-    // line l0 + (lane mod G), elements lane div G + q (64 / G) (G a power of
-    // two); with the ping-pong transform, the HBM mapping above.
     const int s0 = MODE == 0 ? sign : -1;
-    constexpr bool wave = XP == kXWave;  // every frame of the launch has this x_path()
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int l0 = 0, G = 0;
-    wave_share(C, kBlock / 64, wv, l0, G);
-    cplx<T> *res = lines;
-    if (wave) {
-        if (G > 0) wave_lines_fft<T, EPT, true>(lines, l0, G, LS, 1, p, twl, s0);
-    } else {
-        FastDiv dC;
-        dC.d = C;
-        dC.m = 1;  // power of two: n div C = n >> lg
-        dC.s = lg;
-        res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
-    }
+    const FastDiv dC = {(unsigned)C, 1u, (unsigned)lg};  // power of two: n div C = n >> lg
+    cplx<T> *res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
     if (MODE == 2) {
-        int cl = c, ie0 = e0, ies = es;
-        bool own = true;
-        if (wave) {
-            own = G > 0;
-            const int lgG = G > 1 ? 31 - __clz(G) : 0;
-            cl = l0 + (lane & (G - 1));
-            ie0 = lane >> lgG;
-            ies = 64 >> lgG;
-        }
+        const int cl = c, ie0 = e0, ies = es;
         const int f = b * C + cl;
-        const bool valid = own && f < P;
+        const bool valid = f < P;
         const int fv = valid ? f : P - 1;
         const int ky = fdiv(fv, F.dldim), kz = fv - ky * ldim;
         const double ibyz = F.iB1[ky] * F.iB2[kz];
@@ -476,16 +450,9 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
                 }
             }
         }
-        if (wave) {
-            wave_lds_sync();
-            if (G > 0) wave_lines_fft<T, EPT, true>(lines, l0, G, LS, 1, p, twl, +1);
-        } else {
-            __syncthreads();
-            res = x_transform<T, XP == kXGeneric>(res, res == lines ? lines2 : lines, C,
-                                                  FastDiv{(unsigned)C, 1u, (unsigned)lg}, LS,
-                                                  p, twl, +1);
-        }
         __syncthreads();
+        res = x_transform<T, XP == kXGeneric>(res, res == lines ? lines2 : lines, C, dC, LS, p,
+                                              twl, +1);
         const bool vs = b * C + c < P;
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {

@@ -40,15 +40,13 @@ constexpr size_t kSlabLds = 159 * 1024;
 
 __host__ __device__ inline int slab_rows(int K1) { return (K1 + 1) & ~1; }
 
-// The z lines (row pairs) and the y lines (columns) are dealt to the
-// workgroup's wavefronts; a wavefront's share of elements per lane decides
-// the variant (small or large, compiled separately; 0: the slab does not
-// apply).
+// The z lines (row pairs) and the y lines (columns) are shared by the
+// workgroup's threads; the elements per thread decide the variant (small or
+// large, compiled separately; 0: the slab does not apply).
 template <typename T> __host__ __device__ inline int slab_ept_for(int K1, int K2, int ldim) {
-    constexpr int W = SlabCfg<T>::block / 64;
+    constexpr int B = SlabCfg<T>::block;
     const int npair = (K1 + 1) / 2;
-    const int gz = (npair + W - 1) / W, gy = (ldim + W - 1) / W;
-    const int need = (max(gz * K2, gy * K1) + 63) / 64;
+    const int need = (max(npair * K2, ldim * K1) + B - 1) / B;
     return need <= SlabCfg<T>::small ? SlabCfg<T>::small
                                      : need <= SlabCfg<T>::large ? SlabCfg<T>::large : 0;
 }
@@ -70,7 +68,7 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
     __attribute__((amdgpu_waves_per_eu(4)))
     fft_yz_fwd_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int kSlabBlock = SlabCfg<T>::block, kSlabWaves = kSlabBlock / 64;
+    constexpr int kSlabBlock = SlabCfg<T>::block;
     constexpr int U = 8;  // staging registers per round
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -105,28 +103,25 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
         }
     }
     __syncthreads();
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int l0, G;
-    wave_share(npair, kSlabWaves, wv, l0, G);
-    wave_lines_fft<T, EPT, false>(slab, l0, G, 2 * RS, 1, F.plan2, twz, -1);
-    // untangle the wavefront's pairs into rows 2c and 2c+1 (reads first)
+    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, -1);
+    // untangle the pairs into rows 2c and 2c+1 (all reads before the barrier)
     {
         constexpr int Q = EPT / 2 + 1;
-        const int nk = G * RS;
+        const int nk = npair * RS;
         cplx<T> za[Q], zb[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = max(min(lane + q * 64, nk - 1), 0);
-            const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
+            const int l = min((int)threadIdx.x + q * kSlabBlock, nk - 1);
+            const int c = fdiv(l, F.dldim), k = l - c * RS;
             za[q] = slab[c * 2 * RS + k];
             zb[q] = slab[c * 2 * RS + (k == 0 ? 0 : K2 - k)];
         }
-        wave_lds_sync();
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = lane + q * 64;
+            const int l = (int)threadIdx.x + q * kSlabBlock;
             if (l < nk) {
-                const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
+                const int c = fdiv(l, F.dldim), k = l - c * RS;
                 const cplx<T> zk = za[q], zm = zb[q];
                 cplx<T> a, bb;
                 a.x = T(0.5) * (zk.x + zm.x);
@@ -140,9 +135,7 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
         __syncthreads();
     }
     // y transform: ldim lines (columns kz), elements at stride RS
-    wave_share(RS, kSlabWaves, wv, l0, G);
-    wave_lines_fft<T, EPT, true>(slab, l0, G, 1, RS, F.plan1, twy, -1);
-    __syncthreads();
+    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, -1);
     cplx<T> *out = (cplx<T> *)F.grid + (long)x * K1 * RS;
     for (int l = threadIdx.x; l < K1 * RS; l += kSlabBlock) out[l] = slab[l];
 }
@@ -154,7 +147,7 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
     __attribute__((amdgpu_waves_per_eu(4)))
     fft_yz_inv_kernel(SPME_FRAME_ARGS, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int kSlabBlock = SlabCfg<T>::block, kSlabWaves = kSlabBlock / 64;
+    constexpr int kSlabBlock = SlabCfg<T>::block;
     constexpr int U = 8;
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -197,32 +190,26 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
         }
     }
     __syncthreads();
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int l0, G;
-    wave_share(RS, kSlabWaves, wv, l0, G);
-    wave_lines_fft<T, EPT, true>(slab, l0, G, 1, RS, F.plan1, twy, +1);
-    __syncthreads();
-    // Z_k = A_k + i B_k and the Hermitian completion of the wavefront's
-    // pairs (reads first)
-    wave_share(npair, kSlabWaves, wv, l0, G);
+    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
+    // Z_k = A_k + i B_k and the Hermitian completion (all reads before the barrier)
     {
         constexpr int Q = EPT / 2 + 1;
-        const int nk = G * RS;
+        const int nk = npair * RS;
         const bool even = (K2 % 2) == 0;
         cplx<T> ra[Q], rb[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = max(min(lane + q * 64, nk - 1), 0);
-            const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
+            const int l = min((int)threadIdx.x + q * kSlabBlock, nk - 1);
+            const int c = fdiv(l, F.dldim), k = l - c * RS;
             ra[q] = slab[(2 * c) * RS + k];
             rb[q] = slab[(2 * c + 1) * RS + k];
         }
-        wave_lds_sync();
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int l = lane + q * 64;
+            const int l = (int)threadIdx.x + q * kSlabBlock;
             if (l < nk) {
-                const int c = l0 + fdiv(l, F.dldim), k = l - (c - l0) * RS;
+                const int c = fdiv(l, F.dldim), k = l - c * RS;
                 cplx<T> a = ra[q], bb = rb[q];
                 if (k == 0 || (even && k == K2 / 2)) {
                     a.y = T(0);
@@ -241,10 +228,9 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
                 }
             }
         }
-        wave_lds_sync();
+        __syncthreads();
     }
-    wave_lines_fft<T, EPT, false>(slab, l0, G, 2 * RS, 1, F.plan2, twz, +1);
-    __syncthreads();
+    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, +1);
     T *g = (T *)F.grid + (long)x * K1 * 2 * RS;
     const int nz = npair * K2;
     for (int l = threadIdx.x; l < nz; l += kSlabBlock) {
