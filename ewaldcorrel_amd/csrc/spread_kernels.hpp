@@ -212,6 +212,8 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     __shared__ int seg_begin[kMaxSeg];
     __shared__ int seg_cnt[kMaxSeg];
     __shared__ int seg_pref[kMaxSeg + 1];
+    __shared__ int seg_ilo[kMaxSeg / 2], seg_np[kMaxSeg / 2];
+    __shared__ unsigned seg_inv[kMaxSeg / 2];
     int b;
     const FrameDesc &F = frames[find_frame(offs, nf, b)];
     const int n = order_of<ORDER>(F.order);
@@ -228,8 +230,12 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
 
     // column segments of the neighbourhood: for each cx, the cy range
     // [y0 - n/2, y0 + tyr + n/2 - 2] (mod K1) is one or two runs of
-    // consecutive column ids, i.e. one or two runs of sorted atoms
+    // consecutive column ids, i.e. one or two runs of sorted atoms.  An atom
+    // in neighbourhood row d (cx = x0 - n/2 + d) reaches the tile with its
+    // planes i in [d - txr + 1, d] only; when the neighbourhood wraps onto
+    // itself (K0 < txr + n - 1) every plane is tried.
     const int ncx = min(txr + n - 1, K0), ncy = min(tyr + n - 1, K1);
+    const bool wrapped = ncx < txr + n - 1;
     if ((int)threadIdx.x < ncx) {  // segments 2d and 2d+1 of column row d
         const int d = threadIdx.x;
         const int cy0 = imod(y0 - n / 2, K1);
@@ -242,10 +248,16 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
             c0 = F.colstart[cx * K1];
             c1 = F.colstart[cx * K1 + (ncy - len0)];
         }
+        const int ilo = wrapped ? 0 : max(d - txr + 1, 0);
+        const int ihi = wrapped ? n - 1 : min(d, n - 1);
+        const int np = ihi - ilo + 1;  // >= 1
         seg_begin[2 * d] = a0;
-        seg_cnt[2 * d] = a1 - a0;
+        seg_cnt[2 * d] = (a1 - a0) * np;
         seg_begin[2 * d + 1] = c0;
-        seg_cnt[2 * d + 1] = c1 - c0;
+        seg_cnt[2 * d + 1] = (c1 - c0) * np;
+        seg_ilo[d] = ilo;
+        seg_np[d] = np;
+        seg_inv[d] = np > 1 ? 0xffffffffu / (unsigned)np + 1u : 0u;  // ceil(2^32 / np)
     }
     __syncthreads();
     const int nseg = 2 * ncx;
@@ -258,7 +270,7 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         seg_pref[nseg] = tot;
     }
     __syncthreads();
-    const int units = seg_pref[nseg] * n;
+    const int units = seg_pref[nseg];
     const double scale = FIXED ? *F.fxscale : 1.0;
     const AtomRec<T> *recs = (const AtomRec<T> *)F.recs;
 
@@ -274,15 +286,21 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
             const int u = base + q * (int)blockDim.x + (int)threadIdx.x;
             pos[q] = -1;
             if (u < units) {
-                const int la = u / n;
-                iu[q] = u - la * n;
                 int lo = 0, hi = nseg - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (seg_pref[mid] <= la) lo = mid;
+                    if (seg_pref[mid] <= u) lo = mid;
                     else hi = mid - 1;
                 }
-                pos[q] = seg_begin[lo] + (la - seg_pref[lo]);
+                // off = a * np + p: atom a of the segment, its plane ilo + p
+                const int d = lo >> 1, off = u - seg_pref[lo];
+                // multiply-high by ceil(2^32/np) (np >= 2) divides exactly below 2^24
+                const int np = seg_np[d];
+                const int a = np == 1             ? off
+                              : off < (1 << 24) ? (int)__umulhi((unsigned)off, seg_inv[d])
+                                                : off / np;
+                iu[q] = seg_ilo[d] + (off - a * seg_np[d]);
+                pos[q] = seg_begin[lo] + a;
             }
         }
         // unconditional loads (index 0 is always allocated) keep them all in flight
