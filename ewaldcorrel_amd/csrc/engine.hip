@@ -232,6 +232,36 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     return ok;
 }
 
+// Spreading tiles may use more than the default 64 KB of dynamic LDS: two
+// 1024-thread workgroups of up to kSpreadLds each still share one CU.  The
+// attribute is set once for every instantiation; without it the tiles stay
+// within 64 KB.
+constexpr size_t kSpreadLds = 76 * 1024;
+template <typename T, int ORDER> bool spread_attr() {
+    bool ok = hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kSpreadLds) == hipSuccess;
+    if (!std::is_same<T, float>::value)
+        ok = ok && hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kSpreadLds) == hipSuccess;
+    return ok;
+}
+template <typename T> size_t spread_lds_budget() {
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        int dev = 0, maxsh = 0;
+        ok = hipGetDevice(&dev) == hipSuccess &&
+             hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+                 hipSuccess &&
+             (size_t)maxsh >= kSpreadLds && spread_attr<T, 4>() && spread_attr<T, 6>() &&
+             spread_attr<T, 8>() && spread_attr<T, 0>();
+        if (!ok) (void)hipGetLastError();
+    });
+    return ok ? kSpreadLds : kLdsBudget;
+}
+
 }  // namespace
 
 StageSeq eval_stages(bool slab) { return slab ? seq_of(kEvalSlab) : seq_of(kEvalLines); }
@@ -363,10 +393,13 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
     }
+    // spreading tiles: rows of K2 + order - 1 cells (the z stencil unwrapped)
     static const int tiles[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+    const size_t spread_budget =
+        fp32 ? spread_lds_budget<float>() : spread_lds_budget<double>();
     for (auto &t : tiles) {
         const int tx = std::min(t[0], K[0]), ty = std::min(t[1], K[1]);
-        if ((size_t)tx * ty * K[2] * 8 <= kLdsBudget) {
+        if ((size_t)tx * ty * (K[2] + order - 1) * 8 <= spread_budget) {
             D.tx = tx;
             D.ty = ty;
             break;
@@ -584,7 +617,7 @@ int Engine::stage_blocks(Stage s) const {
 size_t Engine::stage_lds(Stage s) const {
     const FrameDesc &D = desc_;
     switch (s) {
-        case ST_SPREAD: return (size_t)D.tx * D.ty * K[2] * 8;
+        case ST_SPREAD: return (size_t)D.tx * D.ty * (K[2] + order - 1) * 8;
 
         case ST_ZR2C:
         case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);

@@ -203,17 +203,28 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(SPME_FRAME_ARGS) {
 }
 
 // ---- spread: one workgroup per x-y tile -------------------------------------------------------
+// LDS holds the tile's full-z columns as rows of Kp = K2 + n - 1 cells: an atom's z
+// stencil z = cz + n/2 - k is stored unwrapped at zz = cz + n - 1 - k, so every
+// stencil line is n consecutive cells, and the write-out folds zz back onto
+// (zz - n/2 + 1) mod K2.  One thread per atom of the neighbourhood: its record and
+// splines are loaded once, then its planes and rows that fall in the tile add
+// their lines.
 constexpr int kMaxSeg = 64;
+
+// (int)floor(x + 0.5) in one instruction; ties are irrelevant here (any fixed rule
+// keeps the sums deterministic)
+__device__ __forceinline__ int round_half_up(float x) {
+    int q;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(x));
+    return q;
+}
 
 template <typename T, int ORDER, bool FIXED>
 __global__ void __launch_bounds__(kSpreadBlock)
 spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int seg_begin[kMaxSeg];
-    __shared__ int seg_cnt[kMaxSeg];
     __shared__ int seg_pref[kMaxSeg + 1];
-    __shared__ int seg_ilo[kMaxSeg / 2], seg_np[kMaxSeg / 2];
-    __shared__ unsigned seg_inv[kMaxSeg / 2];
     int b;
     const FrameDesc &F = frames[find_frame(offs, nf, b)];
     const int n = order_of<ORDER>(F.order);
@@ -222,180 +233,173 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     const int X = b % F.ntx, Y = b / F.ntx;
     const int x0 = X * F.tx, y0 = Y * F.ty;
     const int txr = min(F.tx, K0 - x0), tyr = min(F.ty, K1 - y0);
-    const int cells = txr * tyr * K2;
+    const int Kp = K2 + n - 1;
+    const int cells = txr * tyr * Kp;
 
     typedef typename std::conditional<FIXED, unsigned long long, double>::type Acc;
     Acc *acc = reinterpret_cast<Acc *>(smem_raw);
-    for (int i = threadIdx.x; i < cells; i += blockDim.x) acc[i] = Acc(0);
-
-    // column segments of the neighbourhood: for each cx, the cy range
-    // [y0 - n/2, y0 + tyr + n/2 - 2] (mod K1) is one or two runs of
-    // consecutive column ids, i.e. one or two runs of sorted atoms.  An atom
-    // in neighbourhood row d (cx = x0 - n/2 + d) reaches the tile with its
-    // planes i in [d - txr + 1, d] only; when the neighbourhood wraps onto
-    // itself (K0 < txr + n - 1) every plane is tried.
-    const int ncx = min(txr + n - 1, K0), ncy = min(tyr + n - 1, K1);
-    const bool wrapped = ncx < txr + n - 1;
-    if ((int)threadIdx.x < ncx) {  // segments 2d and 2d+1 of column row d
-        const int d = threadIdx.x;
-        const int cy0 = imod(y0 - n / 2, K1);
-        const int len0 = min(ncy, K1 - cy0);
-        const int cx = imod(x0 - n / 2 + d, K0);
-        const int col = cx * K1 + cy0;
-        const int a0 = F.colstart[col], a1 = F.colstart[col + len0];
-        int c0 = 0, c1 = 0;
-        if (len0 < ncy) {
-            c0 = F.colstart[cx * K1];
-            c1 = F.colstart[cx * K1 + (ncy - len0)];
-        }
-        const int ilo = wrapped ? 0 : max(d - txr + 1, 0);
-        const int ihi = wrapped ? n - 1 : min(d, n - 1);
-        const int np = ihi - ilo + 1;  // >= 1
-        seg_begin[2 * d] = a0;
-        seg_cnt[2 * d] = (a1 - a0) * np;
-        seg_begin[2 * d + 1] = c0;
-        seg_cnt[2 * d + 1] = (c1 - c0) * np;
-        seg_ilo[d] = ilo;
-        seg_np[d] = np;
-        seg_inv[d] = np > 1 ? 0xffffffffu / (unsigned)np + 1u : 0u;  // ceil(2^32 / np)
+    {
+        uint4 *z4 = reinterpret_cast<uint4 *>(smem_raw);
+        for (int i = threadIdx.x; i < cells / 2; i += blockDim.x) z4[i] = make_uint4(0, 0, 0, 0);
+        if ((cells & 1) && threadIdx.x == 0) acc[cells - 1] = Acc(0);
     }
-    __syncthreads();
+
+    // column segments of the neighbourhood: for each cx-row d (cx = x0 - n/2 + d),
+    // the cy range [y0 - n/2, y0 - n/2 + ncy) (mod K1) is one or two runs of
+    // consecutive column ids, i.e. one or two runs of sorted atoms (segments 2d,
+    // 2d+1).  The first wavefront builds them and scans their sizes.
+    const int ncx = min(txr + n - 1, K0), ncy = min(tyr + n - 1, K1);
+    const bool wrapped = ncx < txr + n - 1 || ncy < tyr + n - 1;
     const int nseg = 2 * ncx;
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int s = 0; s < nseg; ++s) {
-            seg_pref[s] = tot;
-            tot += seg_cnt[s];
+    if (threadIdx.x < 64) {
+        const int s = threadIdx.x;
+        int beg = 0, cnt = 0;
+        if (s < nseg) {
+            const int d = s >> 1;
+            const int cy0 = imod(y0 - n / 2, K1);
+            const int len0 = min(ncy, K1 - cy0);
+            const int cx = imod(x0 - n / 2 + d, K0);
+            if ((s & 1) == 0) {
+                beg = F.colstart[cx * K1 + cy0];
+                cnt = F.colstart[cx * K1 + cy0 + len0] - beg;
+            } else if (len0 < ncy) {
+                beg = F.colstart[cx * K1];
+                cnt = F.colstart[cx * K1 + (ncy - len0)] - beg;
+            }
         }
-        seg_pref[nseg] = tot;
+        int v = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if (s >= o) v += t;
+        }
+        if (s < nseg) {
+            seg_begin[s] = beg;
+            seg_pref[s] = v - cnt;
+        }
+        if (s == 63) seg_pref[nseg] = v;
     }
     __syncthreads();
     const int units = seg_pref[nseg];
     const double scale = FIXED ? *F.fxscale : 1.0;
-    const AtomRec<T> *recs = (const AtomRec<T> *)F.recs;
+    typedef __attribute__((address_space(1))) const AtomRec<T> GRec;
+    GRec *recs = (GRec *)F.recs;
 
-    // one unit = (atom, i): the x-plane of its stencil.  UPT units per thread
-    // per round: their records are loaded together, then processed.
-    constexpr int UPT = 4;
-    for (int base = 0; base < units; base += UPT * (int)blockDim.x) {
-        int pos[UPT], iu[UPT], cell[UPT];
-        T fx[UPT], fy[UPT], fz[UPT];
-        double wv[UPT];
-#pragma unroll
-        for (int q = 0; q < UPT; ++q) {
-            const int u = base + q * (int)blockDim.x + (int)threadIdx.x;
-            pos[q] = -1;
-            if (u < units) {
-                int lo = 0, hi = nseg - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (seg_pref[mid] <= u) lo = mid;
-                    else hi = mid - 1;
-                }
-                // off = a * np + p: atom a of the segment, its plane ilo + p
-                const int d = lo >> 1, off = u - seg_pref[lo];
-                // multiply-high by ceil(2^32/np) (np >= 2) divides exactly below 2^24
-                const int np = seg_np[d];
-                const int a = np == 1             ? off
-                              : off < (1 << 24) ? (int)__umulhi((unsigned)off, seg_inv[d])
-                                                : off / np;
-                iu[q] = seg_ilo[d] + (off - a * seg_np[d]);
-                pos[q] = seg_begin[lo] + a;
-            }
+    for (int u = threadIdx.x; u < units; u += blockDim.x) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (seg_pref[mid] <= u) lo = mid;
+            else hi = mid - 1;
         }
-        // unconditional loads (index 0 is always allocated) keep them all in flight
-#pragma unroll
-        for (int q = 0; q < UPT; ++q) {
-            const AtomRec<T> &r = recs[pos[q] >= 0 ? pos[q] : 0];
-            cell[q] = r.cell;
-            fx[q] = r.f[0];
-            fy[q] = r.f[1];
-            fz[q] = r.f[2];
-            wv[q] = r.w;
-        }
-#pragma unroll
-        for (int q = 0; q < UPT; ++q) {
-            if (pos[q] < 0) continue;
-            const int i = iu[q];
-            const int cx = cell[q] >> 20;
-            const int cy = (cell[q] >> 10) & 1023;
-            const int cz = cell[q] & 1023;
-            const int lx = imod(cx + n / 2 - i, K0) - x0;
-            if (lx < 0 || lx >= txr) continue;
-            T by[CAP], bz[CAP];
-            spline_w<T, ORDER>(sp, n, fy[q], by);
-            spline_w<T, ORDER>(sp, n, fz[q], bz);
-            // z and y targets once per unit; one conditional wrap suffices when
-            // the stencil is no longer than the grid, imod otherwise
-            int zt[CAP], yt[CAP];
+        const int d = lo >> 1;
+        GRec &r = recs[seg_begin[lo] + (u - seg_pref[lo])];
+        const int cell = r.cell;
+        const int cx = cell >> 20;
+        const int cy = (cell >> 10) & 1023;
+        const int cz = cell & 1023;
+        T bx[CAP], by[CAP], bz[CAP];
+        spline_w<T, ORDER>(sp, n, r.f[0], bx);
+        spline_w<T, ORDER>(sp, n, r.f[1], by);
+        spline_w<T, ORDER>(sp, n, r.f[2], bz);
+        // FP32: int32 contributions of the pre-scaled charge (|w scale| Mmax^3 < 2^30
+        // keeps every product in range); FP64 fixed: int64 of the scaled product;
+        // FP64 float: the product itself
+        typedef typename std::conditional<std::is_same<T, float>::value && FIXED, float,
+                                          double>::type W;
+        const W ws = W(r.w * scale);
+        auto line = [&](Acc *row, W wxy) {
+            // row points at zz = cz; stencil point k lands at row[n - 1 - k]
 #pragma unroll
             for (int k = 0; k < CAP; ++k) {
-                int z = cz + n / 2 - k;
-                if (K2 >= n) z = z < 0 ? z + K2 : (z >= K2 ? z - K2 : z);
-                else z = imod(z, K2);
-                zt[k] = z;
-                int y = cy + n / 2 - k;
-                if (K1 >= n) y = y < 0 ? y + K1 : (y >= K1 ? y - K1 : y);
-                else y = imod(y, K1);
-                yt[k] = y - y0;
-            }
-            if (std::is_same<T, float>::value && FIXED) {
-                // FP32 path: the scaled x factor (|wx| < 2^30, formed in FP64 since
-                // the scale alone may exceed the FP32 range), FP32 products, int32
-                // contributions
-                const float wx =
-                    (float)(wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i) * scale);
-#pragma unroll
-                for (int j = 0; j < CAP; ++j) {
-                    if (j >= n) break;
-                    const int ly = yt[j];
-                    if (ly < 0 || ly >= tyr) continue;
-                    const float wxy = wx * (float)by[j];
-                    Acc *row = acc + (lx * tyr + ly) * K2;
-#pragma unroll
-                    for (int k = 0; k < CAP; ++k) {
-                        if (k >= n) break;
-                        const int qv = __float2int_rn(wxy * (float)bz[k]);
-                        atomicAdd((unsigned long long *)&row[zt[k]],
-                                  (unsigned long long)(long long)qv);
-                    }
+                if (k >= n) break;
+                if (std::is_same<W, float>::value) {
+                    const int qv = round_half_up((float)wxy * (float)bz[k]);
+                    atomicAdd((unsigned long long *)&row[n - 1 - k],
+                              (unsigned long long)(long long)qv);
+                } else if (FIXED) {
+                    const long long qv = llrint((double)wxy * (double)bz[k]);
+                    atomicAdd((unsigned long long *)&row[n - 1 - k], (unsigned long long)qv);
+                } else {
+                    atomicAdd((double *)&row[n - 1 - k], (double)wxy * (double)bz[k]);
                 }
-            } else {
-                const double wx = wv[q] * (double)spline_one<T, ORDER>(sp, n, fx[q], i);
+            }
+        };
+        if (!wrapped) {
+            // the column's place in the neighbourhood; plane i lands on lx = d - i,
+            // row j on ly = dy - j
+            int dy = cy - y0 + n / 2;
+            if (dy < 0) dy += K1;
+            else if (dy >= K1) dy -= K1;
+            // the line of plane i, row j starts (i tyr + j) Kp cells before the
+            // line of plane 0, row 0: a workgroup-uniform offset
+            Acc *base = acc + ((d * tyr + dy) * Kp + cz);
+            const int sx = tyr * Kp;
+#pragma unroll
+            for (int i = 0; i < CAP; ++i) {
+                if (i >= n) break;
+                if ((unsigned)(d - i) >= (unsigned)txr) continue;
+                const W wx = ws * W(bx[i]);
 #pragma unroll
                 for (int j = 0; j < CAP; ++j) {
                     if (j >= n) break;
-                    const int ly = yt[j];
-                    if (ly < 0 || ly >= tyr) continue;
-                    const double wxy = wx * (double)by[j];
-                    Acc *row = acc + (lx * tyr + ly) * K2;
+                    if ((unsigned)(dy - j) >= (unsigned)tyr) continue;
+                    const int off = __builtin_amdgcn_readfirstlane(i * sx + j * Kp);
+                    line(base - off, wx * W(by[j]));
+                }
+            }
+        } else {
+            // the neighbourhood wraps onto itself: every plane and row is tried
 #pragma unroll
-                    for (int k = 0; k < CAP; ++k) {
-                        if (k >= n) break;
-                        if (FIXED) {
-                            const long long qv = llrint(wxy * (double)bz[k] * scale);
-                            atomicAdd((unsigned long long *)&row[zt[k]], (unsigned long long)qv);
-                        } else {
-                            atomicAdd((double *)&row[zt[k]], wxy * (double)bz[k]);
-                        }
-                    }
+            for (int i = 0; i < CAP; ++i) {
+                if (i >= n) break;
+                const int lx = imod(cx + n / 2 - i, K0) - x0;
+                if ((unsigned)lx >= (unsigned)txr) continue;
+                const W wx = ws * W(bx[i]);
+#pragma unroll
+                for (int j = 0; j < CAP; ++j) {
+                    if (j >= n) break;
+                    const int ly = imod(cy + n / 2 - j, K1) - y0;
+                    if ((unsigned)ly >= (unsigned)tyr) continue;
+                    line(acc + ((lx * tyr + ly) * Kp + cz), wx * W(by[j]));
                 }
             }
         }
     }
     __syncthreads();
-    // write the tile once (coalesced along z)
-    T *grid = (T *)F.grid;
+    // write the tile once: one wavefront per row, lanes along z (coalesced), the
+    // halo folded back
+    typedef __attribute__((address_space(1))) T GT;
+    GT *grid = (GT *)F.grid;
     const double inv = FIXED ? 1.0 / scale : 1.0;
     const long rs = 2L * F.ldim;
-    for (int idx = threadIdx.x; idx < cells; idx += blockDim.x) {
-        const int r = idx / K2;
-        const int z = idx - r * K2;
-        const int lx = r / tyr, ly = r - lx * tyr;
-        double v;
-        if (FIXED) v = (double)(long long)acc[idx] * inv;
-        else v = (double)acc[idx];
-        grid[((long)(x0 + lx) * K1 + (y0 + ly)) * rs + z] = T(v);
+    const int rows = txr * tyr;
+    const int H = n / 2 - 1;
+    const int lane = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
+    for (int rr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); rr < rows; rr += nw) {
+        const int lx = rr / tyr, ly = rr - lx * tyr;
+        const Acc *arow = acc + rr * Kp;
+        GT *grow = grid + ((long)(x0 + lx) * K1 + (y0 + ly)) * rs;
+        for (int z0 = 0; z0 < K2; z0 += 64) {
+            const int z = z0 + lane;
+            if (z >= K2) continue;
+            int zz = z + H;
+            Acc v;
+            if (K2 >= n) {  // at most one image of the halo per cell
+                if (zz >= K2) zz -= K2;
+                v = arow[zz];
+                if (zz < n - 1) v += arow[zz + K2];
+            } else {
+                zz %= K2;
+                v = arow[zz];
+                for (zz += K2; zz < Kp; zz += K2) v += arow[zz];
+            }
+            double o;
+            if (FIXED) o = (double)(long long)v * inv;
+            else o = (double)v;
+            grow[z] = T(o);
+        }
     }
 }
 
