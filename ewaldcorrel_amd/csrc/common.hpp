@@ -127,6 +127,7 @@ struct FrameDesc {
     int fixed;
     int fx32;        // FP32 spreading: int32 contributions (|q| < 2^30) into int64 sums
     int tx, ty, ntx, nty;  // spreading tiles
+    int gtx, gty, gntx, gnty;  // gather tiles (gtx = 0: no tiled gather)
     // FFT line blocking
     int zC;          // complex lines (two real rows each) per z workgroup
     int yC, yT;      // lines per y tile, tiles over ldim

@@ -45,7 +45,7 @@ const char *stage_name(Stage s) {
         "prep",      "scan",      "scatter",   "spread",    "fft_z_r2c",
         "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
         "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
-        "finish",    "fft_yz_fwd", "fft_yz_inv"};
+        "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile"};
     return names[s];
 }
 
@@ -169,6 +169,14 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_GATHER_SORTED:
             gather_kernel<T, ORDER, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf, sp);
             break;
+        case ST_GATHER_TILE:
+            switch (L.gmz) {
+                case 1: gather_tile_kernel<T, ORDER, 1><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
+                case 2: gather_tile_kernel<T, ORDER, 2><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
+                case 3: gather_tile_kernel<T, ORDER, 3><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
+                default: gather_tile_kernel<T, ORDER, 4><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
+            }
+            break;
         default: break;
     }
     hip_check(hipGetLastError(), stage_name(s));
@@ -202,6 +210,10 @@ const Stage kEvalLines[] = {ST_PREP, ST_SCAN,   ST_SCATTER, ST_SPREAD, ST_ZR2C,
                             ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_SORTED};
 const Stage kEvalSlab[] = {ST_PREP,   ST_SCAN,   ST_SCATTER, ST_SPREAD,
                            ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_SORTED};
+const Stage kEvalLinesTile[] = {ST_PREP, ST_SCAN,   ST_SCATTER, ST_SPREAD, ST_ZR2C,
+                                ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_TILE};
+const Stage kEvalSlabTile[] = {ST_PREP,   ST_SCAN,   ST_SCATTER, ST_SPREAD,
+                               ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_TILE};
 
 template <size_t NS> StageSeq seq_of(const Stage (&a)[NS]) { return StageSeq{a, (int)NS}; }
 
@@ -237,10 +249,21 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
 // attribute is set once for every instantiation; without it the tiles stay
 // within 64 KB.
 constexpr size_t kSpreadLds = 76 * 1024;
+// LDS of a gather tile: (tx + n) x (ty + n) rows of K2 + n - 1 reals
+size_t gather_tile_lds(int tx, int ty, int K2, int n, size_t tsz) {
+    return (size_t)(tx + n) * (ty + n) * (K2 + n - 1) * tsz;
+}
+template <typename T, int ORDER, int MZ> bool gather_attr() {
+    return hipFuncSetAttribute((const void *)gather_tile_kernel<T, ORDER, MZ>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kSpreadLds) == hipSuccess;
+}
 template <typename T, int ORDER> bool spread_attr() {
     bool ok = hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kSpreadLds) == hipSuccess;
+                                  (int)kSpreadLds) == hipSuccess &&
+              gather_attr<T, ORDER, 1>() && gather_attr<T, ORDER, 2>() &&
+              gather_attr<T, ORDER, 3>() && gather_attr<T, ORDER, 4>();
     if (!std::is_same<T, float>::value)
         ok = ok && hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -264,7 +287,10 @@ template <typename T> size_t spread_lds_budget() {
 
 }  // namespace
 
-StageSeq eval_stages(bool slab) { return slab ? seq_of(kEvalSlab) : seq_of(kEvalLines); }
+StageSeq eval_stages(bool slab, bool tile) {
+    if (tile) return slab ? seq_of(kEvalSlabTile) : seq_of(kEvalLinesTile);
+    return slab ? seq_of(kEvalSlab) : seq_of(kEvalLines);
+}
 StageSeq sfac_stages(bool slab) { return slab ? seq_of(kSfacSlab) : seq_of(kSfacLines); }
 StageSeq de1_stages(bool slab) { return slab ? seq_of(kDe1Slab) : seq_of(kDe1Lines); }
 
@@ -408,6 +434,24 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     if (D.tx == 0) throw std::invalid_argument("K2 too large for the spreading tile");
     D.ntx = (K[0] + D.tx - 1) / D.tx;
     D.nty = (K[1] + D.ty - 1) / D.ty;
+    // gather tiles (FP32): the staged block must fit the same LDS budget, its rows
+    // must fit the per-lane row loads, and a tile of fewer than 16 columns stages
+    // too much halo per atom (the per-atom gather is used then; FP64 rows double
+    // the LDS and register footprint, and the per-atom gather is faster there)
+    if (fp32 && K[2] + order - 1 <= 64 * kGatherMaxZ) {
+        static const int gtiles[][2] = {{8, 8}, {8, 4}, {4, 4}};
+        for (auto &t : gtiles) {
+            const int tx = std::min(t[0], K[0]), ty = std::min(t[1], K[1]);
+            if (tx * ty >= std::min(16, K[0] * K[1]) &&
+                gather_tile_lds(tx, ty, K[2], order, tsz) <= spread_budget) {
+                D.gtx = tx;
+                D.gty = ty;
+                D.gntx = (K[0] + tx - 1) / tx;
+                D.gnty = (K[1] + ty - 1) / ty;
+                break;
+            }
+        }
+    }
 
 
     nred_infl_ = (int)std::min<size_t>((H() + kBlock - 1) / kBlock, 1024);
@@ -610,6 +654,7 @@ int Engine::stage_blocks(Stage s) const {
             const int apb = kBlock / order;
             return (D.n + apb - 1) / apb;
         }
+        case ST_GATHER_TILE: return D.gntx * D.gnty;
         default: return 0;
     }
 }
@@ -618,6 +663,8 @@ size_t Engine::stage_lds(Stage s) const {
     const FrameDesc &D = desc_;
     switch (s) {
         case ST_SPREAD: return (size_t)D.tx * D.ty * (K[2] + order - 1) * 8;
+        case ST_GATHER_TILE:
+            return gather_tile_lds(D.gtx, D.gty, K[2], order, fp32 ? 4 : 8);
 
         case ST_ZR2C:
         case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);
@@ -659,7 +706,7 @@ void Engine::run_(Stage s, int finish) {
     if (nb <= 0) return;
     sync_desc_();
     ProfScope ps(this, stage_name(s));
-    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_)}, nb, stage_lds(s), stream_,
+    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz()}, nb, stage_lds(s), stream_,
            finish);
 }
 
@@ -716,7 +763,7 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
             sync_desc_();  // the descriptor upload stays outside the graph
             HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             try {
-                run_seq_(eval_stages(slab()));
+                run_seq_(eval_stages(slab(), gtile()));
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream_, &g);
@@ -731,7 +778,7 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
         }
         HIPC(hipGraphLaunch(graph_exec_, stream_));
     } else {
-        run_seq_(eval_stages(slab()));
+        run_seq_(eval_stages(slab(), gtile()));
     }
     state = 1;
 }
@@ -1039,8 +1086,11 @@ Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
     descs_.resize(nf);
     for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
     slab_ = true;
+    gtile_ = true;
     for (const Engine *e : frames_) {
         slab_ = slab_ && e->slab();
+        gtile_ = gtile_ && e->gtile();
+        gmz_ = std::max(gmz_, e->gmz());
         slab_ept_ = std::max(slab_ept_, e->slab_ept());
         xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
     }
@@ -1105,10 +1155,10 @@ void Batch::evaluate(hipStream_t st) {
     Engine *e0 = frames_[0];
     auto run_all = [&]() {
         Stage prev = ST_COUNT;
-        const StageSeq seq = eval_stages(slab_);
+        const StageSeq seq = eval_stages(slab_, gtile_);
         for (int i = 0; i < seq.n; ++i) {
             const Stage s = seq.s[i];
-            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_};
+            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
             e0->prof_end_on(st);

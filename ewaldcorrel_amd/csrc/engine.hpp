@@ -26,7 +26,7 @@ enum EngineFlags {
 enum Stage {
     ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
     ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
-    ST_YZ_FWD, ST_YZ_INV, ST_COUNT
+    ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_COUNT
 };
 
 // stage sequences of the entry points for the slab / line transforms
@@ -34,7 +34,7 @@ struct StageSeq {
     const Stage *s;
     int n;
 };
-StageSeq eval_stages(bool slab);
+StageSeq eval_stages(bool slab, bool tile);
 StageSeq sfac_stages(bool slab);
 StageSeq de1_stages(bool slab);
 const char *stage_name(Stage s);
@@ -47,6 +47,7 @@ struct Launch {
     int nf;
     int slab_ept;  // elements per thread the slab kernels are compiled for
     int xpath;     // transform path of the x launches (XPath)
+    int gmz;       // row chunks of 64 per lane of the tiled gather (ceil((K2 + n - 1) / 64))
 };
 
 struct FftPlan {
@@ -115,6 +116,9 @@ public:
     bool use_slab = false;
     bool slab() const { return slab_ok && use_slab; }
     int slab_ept() const;
+    // the evaluation gathers through LDS-staged tiles when one fits (FrameDesc::gtx)
+    bool gtile() const { return desc_.gtx > 0; }
+    int gmz() const { return (K[2] + order - 1 + 63) / 64; }
 
     // Per-kernel timing with HIP events recorded on stream() around every
     // launch while enabled (graph replay is off meanwhile).
@@ -243,6 +247,8 @@ private:
     std::vector<size_t> lds_;               // max LDS per stage
     bool dirty_ = true;
     bool slab_ = false;                     // every frame runs the slab transforms
+    bool gtile_ = false;                    // every frame has gather tiles
+    int gmz_ = 1;                           // largest row chunk count of the frames
     int slab_ept_ = 0;                      // slab kernel variant (largest frame)
     int xpath_ = 0;                         // x transform path valid for every frame
     hipGraphExec_t graph_exec_ = nullptr;

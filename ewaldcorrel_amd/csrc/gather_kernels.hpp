@@ -172,4 +172,185 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
     if (F.phi) F.phi[a] = ph * c.iV;
 }
 
+// ---- tiled gather: one workgroup per x-y tile of columns ----------------------------
+// The atoms of a tile's columns (contiguous runs of the column-sorted records,
+// one per cx) read grid rows of the (tx + n) x (ty + n) block around it only.
+// That block, each row unwrapped to K2 + n - 1 reals (position P holds z =
+// (P - n/2) mod K2), is staged in LDS with coalesced row loads; then one thread
+// per atom sums its whole stencil from LDS: the n^2 scattered global reads per
+// atom become LDS reads, and each grid row is fetched from L2 once per tile.
+constexpr int kGatherBlock = 512;
+constexpr int kGatherLoadRegs = 24;     // staged values each lane holds before storing
+constexpr int kGatherMaxZ = 4;          // row length K2 + n - 1 <= 64 kGatherMaxZ
+
+// MZ = ceil((K2 + n - 1) / 64) row chunks per lane
+template <typename T, int ORDER, int MZ>
+__global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(4)))
+gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int b;
+    const int bid = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const FrameDesc &F = frames[find_frame_of(offs, nf, bid, b)];
+    const int n = order_of<ORDER>(F.order), h = n / 2;
+    constexpr int CAP = Ord<ORDER>::cap;
+    const int K0 = F.K0, K1 = F.K1, K2 = F.K2;
+    const int X = b % F.gntx, Y = b / F.gntx;
+    const int x0 = X * F.gtx, y0 = Y * F.gty;
+    const int txr = min(F.gtx, K0 - x0), tyr = min(F.gty, K1 - y0);
+    const int PY = tyr + n, Kp = K2 + n - 1;
+    const int rows = (txr + n) * PY;
+    T *tile = reinterpret_cast<T *>(smem_raw);
+
+    // the tile's atoms: cx-row d holds the columns [y0, y0 + tyr) of cx = x0 + d,
+    // one run of records; the first wavefront scans the run lengths into LDS
+    __shared__ int run_begin[64];
+    __shared__ int run_pref[65];
+    if (threadIdx.x < 64) {
+        const int d = threadIdx.x;
+        int beg = 0, cnt = 0;
+        if (d < txr) {
+            const int c = (x0 + d) * K1 + y0;
+            beg = F.colstart[c];
+            cnt = F.colstart[c + tyr] - beg;
+        }
+        int v = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if (d >= o) v += t;
+        }
+        run_begin[d] = beg;
+        run_pref[d] = v - cnt;
+        if (d == 63) run_pref[64] = v;
+    }
+
+    // stage the rows: wavefronts over rows, lanes along z.  Every load and store
+    // is unconditional within a row: a lane past the end of the row repeats its
+    // own previous element (the same value to the same address).  Lane q of a
+    // wavefront holds the source offset of the wavefront's q-th row, read back
+    // with readlane.
+    {
+        typedef __attribute__((address_space(1))) const T GT;
+        const int lane = threadIdx.x & 63;
+        const int nw = blockDim.x >> 6;
+        const int w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        int zsrc[MZ], zdst[MZ];
+#pragma unroll
+        for (int m = 0; m < MZ; ++m) {
+            int zp = lane + 64 * m;
+            if (zp >= Kp) zp = m > 0 ? zdst[m - 1] : Kp - 1;
+            int z = zp - h;
+            if (K2 >= n) z = z < 0 ? z + K2 : (z >= K2 ? z - K2 : z);
+            else z = imod(z, K2);
+            zsrc[m] = z;
+            zdst[m] = zp;
+        }
+        const int nrw = rows > w0 ? (rows - w0 + nw - 1) / nw : 0;  // rows of this wavefront
+        int my_src = 0, my_dst = 0;
+        {
+            const int r = w0 + lane * nw;
+            if (r < rows) {
+                const int a = r / PY, c = r - a * PY;
+                const int gx = imod(x0 - h + a, K0), gy = imod(y0 - h + c, K1);
+                my_src = (gx * K1 + gy) * (2 * F.ldim);
+                my_dst = r * Kp;
+            }
+        }
+        GT *grid = (GT *)F.grid;
+        constexpr int RIF = kGatherLoadRegs / MZ;  // rows in flight per wavefront
+        for (int q0 = 0; q0 < nrw; q0 += RIF) {
+            T v[RIF][MZ];
+#pragma unroll
+            for (int q = 0; q < RIF; ++q) {
+                if (q0 + q < nrw) {  // uniform
+                    GT *src = grid + __builtin_amdgcn_readlane(my_src, q0 + q);
+#pragma unroll
+                    for (int m = 0; m < MZ; ++m) v[q][m] = src[zsrc[m]];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RIF; ++q) {
+                if (q0 + q < nrw) {
+                    T *dst = tile + __builtin_amdgcn_readlane(my_dst, q0 + q);
+#pragma unroll
+                    for (int m = 0; m < MZ; ++m) dst[zdst[m]] = v[q][m];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int units = run_pref[64];
+    typedef __attribute__((address_space(1))) const AtomRec<T> GRec;
+    GRec *recs = (GRec *)F.recs;
+    for (int u = threadIdx.x; u < units; u += blockDim.x) {
+        int lo = 0;
+        for (int d = 1; d < txr; ++d)
+            if (run_pref[d] <= u) lo = d;
+        GRec &r = recs[run_begin[lo] + (u - run_pref[lo])];
+        const int a = r.perm;
+        const double wa = r.w;
+        const int cell = r.cell;
+        const T f0 = r.f[0], f1 = r.f[1], f2 = r.f[2];
+        const int c0 = cell >> 20, c1 = (cell >> 10) & 1023, c2 = cell & 1023;
+        // k = ceil(u): c + 1 for f > 0, c for f = 0; its stencil covers the
+        // staged planes k - x0 .. k - x0 + n - 1 (and likewise in y and z)
+        const int a0 = (f0 > T(0) ? c0 + 1 : c0) - x0;
+        const int b0 = (f1 > T(0) ? c1 + 1 : c1) - y0;
+        int k2 = f2 > T(0) ? c2 + 1 : c2;
+        if (k2 >= K2) k2 -= K2;
+        const T t0 = f0 > T(0) ? T(1) - f0 : T(0);
+        const T t1 = f1 > T(0) ? T(1) - f1 : T(0);
+        const T t2 = f2 > T(0) ? T(1) - f2 : T(0);
+        T bx[CAP], dbx[CAP], by[CAP], dby[CAP], bz[CAP], dbz[CAP];
+        spline_wd<T, ORDER>(sp, n, t0, bx, dbx);
+        spline_wd<T, ORDER>(sp, n, t1, by, dby);
+        spline_wd<T, ORDER>(sp, n, t2, bz, dbz);
+        // row addresses by running sums (the empty asm keeps the compiler from
+        // precomputing all n^2 offsets as loop invariants, which it spills)
+        int plane = (a0 * PY + b0) * Kp + k2;
+        const int step_x = PY * Kp;
+        T e0 = 0, e1 = 0, e2 = 0, ph = 0;
+#pragma unroll
+        for (int i = 0; i < CAP; ++i) {
+            if (i >= n) break;
+            T pb = 0, pdb = 0, pbdz = 0;
+            int row = plane;
+#pragma unroll
+            for (int j = 0; j < CAP; ++j) {
+                if (j >= n) break;
+                T zb = 0, zdb = 0;
+#pragma unroll
+                for (int k = 0; k < CAP; ++k) {
+                    if (k >= n) break;
+                    const T q = tile[row + k];
+                    zb += bz[k] * q;
+                    zdb += dbz[k] * q;
+                }
+                pb += by[j] * zb;
+                pdb += dby[j] * zb;
+                pbdz += by[j] * zdb;
+                row += Kp;
+                asm volatile("" : "+v"(row));
+            }
+            plane += step_x;
+            asm volatile("" : "+v"(plane));
+            e0 += dbx[i] * pb;
+            e1 += bx[i] * pdb;
+            e2 += bx[i] * pbdz;
+            ph += bx[i] * pb;
+        }
+        const CellArgs &cl = F.cell;
+        if (F.dx) {
+            const double f = -wa * cl.iV;
+            const double g0 = (double)e0 * f * K0, g1 = (double)e1 * f * K1,
+                         g2 = (double)e2 * f * K2;
+            F.dx[3 * a + 0] = cl.iL00 * g0;
+            F.dx[3 * a + 1] = cl.iL10 * g0 + cl.iL11 * g1;
+            F.dx[3 * a + 2] = cl.iL20 * g0 + cl.iL21 * g1 + cl.iL22 * g2;
+        }
+        if (F.phi) F.phi[a] = (double)ph * cl.iV;
+    }
+}
+
 }  // namespace spme

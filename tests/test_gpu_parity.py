@@ -269,6 +269,27 @@ def test_fp32_error_bounds(case):
     assert rel_rms(phi, ref[4]) <= 1e-5
 
 
+@pytest.mark.parametrize("order", [4, 6])
+@pytest.mark.parametrize("natoms", [40, 900])
+def test_fp32_tiled_gather_sparse_tiles(order, natoms):
+    """The FP32 evaluation gathers through LDS-staged tiles of 8 x 8 columns.
+    With 40 atoms on a 20 x 24 x 30 grid most tiles hold a handful of atoms
+    (fewer than the tile's column rows), with 900 the last wavefront of a tile is
+    partly idle: forces and potentials against the oracle, and two evaluations
+    on two handles bit-identical."""
+    sysd = make_sys([16., 17., 18., 0.5, 0.2, -0.3], (20, 24, 30), order, natoms, 2,
+                    beta=0.4)
+    ref = run_oracle(sysd)
+    s1, E, vir, dx, phi = run_gpu(sysd, "fp32")
+    s2, E2, vir2, dx2, phi2 = run_gpu(sysd, "fp32")
+    eF, eP = rel_rms(dx, ref[3]), rel_rms(phi, ref[4])
+    say(f"order {order} n {natoms}: relE {abs(E - ref[1]) / abs(ref[1]):.2e} F {eF:.2e} "
+        f"phi {eP:.2e}")
+    assert abs(E - ref[1]) <= 1e-6 * abs(ref[1])
+    assert eF <= 1e-5 and eP <= 1e-5
+    assert np.array_equal(dx, dx2) and np.array_equal(phi, phi2)
+
+
 # ---- determinism of the fixed-point grid --------------------------------------------------
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_fixed_point_grid_is_order_independent(precision):
