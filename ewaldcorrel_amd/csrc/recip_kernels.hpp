@@ -77,55 +77,100 @@ __host__ __device__ inline void ewald_point_impl(int K0, int K1, int K2, const C
     for (int q = 0; q < 6; ++q) dA[q] = mult * d[q];
 }
 
-// FP32 evaluation of the same closed form for the FP32 engine: the wave
-// vector and the ball test in FP64 (so the set of images is that of the
-// FP64 form), then exp, 1/m^2 and the products in FP32.
-__device__ __forceinline__ void ewald_image_f32(const CellArgs &c, float fac, double max_m2,
-                                                int ix, int iy, int iz, float &a, float *d) {
-    const double nx = ix, ny = iy, nz = iz;
-    const double mxd = nx * c.iL00;
-    const double myd = nx * c.iL10 + ny * c.iL11;
-    const double mzd = nx * c.iL20 + ny * c.iL21 + nz * c.iL22;
-    const double m2d = mxd * mxd + myd * myd + mzd * mzd;
-    if (m2d < 1e-10 || m2d > max_m2) return;
-    const float m2 = (float)m2d, mx = (float)mxd, my = (float)myd, mz = (float)mzd;
-    const float r = __builtin_amdgcn_rcpf(m2);
-    const float f = expf(m2 * fac) * r * 0.318309886183790672f;  // 1/pi
-    const float fp = (fac - r) * f;
-    a += f;
-    d[0] += fp * mx * mx;
-    d[1] += fp * my * my;
-    d[2] += fp * mz * mz;
-    d[3] += fp * my * mx;
-    d[4] += fp * mz * mx;
-    d[5] += fp * mz * my;
+// FP32 closed form along one x line (fixed ky, kz) of the fused x pass: the
+// line's parts of the wave vector m = n iL are computed once, so a point costs
+// three products for m; exp by the hardware exp2.  The ball test |m|^2 <= max_m2
+// runs in FP32 and is redone exactly as ewald_image does it (FP64, no
+// contraction) within a relative band of 1e-6 around the cutoff, so the set of
+// images is that of the FP64 form.  Images and factors as ewald_point_impl.
+struct XLineF32 {
+    float iL00, iL10, iL20;  // m per unit nx
+    float Y1, Y2, Z;         // ny iL11, ny iL21, nz iL22
+    float fac, fac2, max_m2f, mult;
+    int K0, half0, ny, nz;
+    bool even0, yny, zn2;
+};
+
+__device__ __forceinline__ void xline_init(XLineF32 &L, const FrameDesc &F, int ky, int kz) {
+    const CellArgs &c = F.tcell;
+    const int K1 = F.K1, K2 = F.K2;
+    L.K0 = F.K0;
+    L.half0 = F.K0 / 2;
+    L.even0 = (F.K0 % 2) == 0;
+    L.ny = ky <= K1 / 2 ? ky : ky - K1;
+    L.nz = kz;
+    L.yny = (K1 % 2) == 0 && ky == K1 / 2;
+    const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
+    L.zn2 = zny && kz > 0;
+    L.mult = (kz == 0 || zny) ? 1.0f : 2.0f;
+    L.iL00 = (float)c.iL00;
+    L.iL10 = (float)c.iL10;
+    L.iL20 = (float)c.iL20;
+    L.Y1 = (float)(L.ny * c.iL11);
+    L.Y2 = (float)(L.ny * c.iL21);
+    L.Z = (float)(kz * c.iL22);
+    L.fac = (float)F.ewald_fac;
+    L.fac2 = (float)(F.ewald_fac * 1.4426950408889634);  // log2(e)
+    L.max_m2f = (float)F.max_m2;
 }
 
-__device__ __forceinline__ void ewald_point_f32(const FrameDesc &F, int kx, int ky, int kz,
-                                                float &A, float *dA) {
-    const int K0 = F.K0, K1 = F.K1, K2 = F.K2;
-    const int nx0 = kx <= K0 / 2 ? kx : kx - K0;
-    const int ny0 = ky <= K1 / 2 ? ky : ky - K1;
-    const bool xny = (K0 % 2) == 0 && kx == K0 / 2;
-    const bool yny = (K1 % 2) == 0 && ky == K1 / 2;
-    const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
-    const bool zn2 = zny && kz > 0;
-    const float mult = (kz == 0 || zny) ? 1.0f : 2.0f;
-    const float fac = (float)F.ewald_fac;
-    float a = 0.0f, d[6] = {0, 0, 0, 0, 0, 0};
-    ewald_image_f32(F.tcell, fac, F.max_m2, nx0, ny0, kz, a, d);
-    if (xny || yny || zn2) {
-        const int cx = xny ? 2 : 1, cy = yny ? 2 : 1, cz = zn2 ? 2 : 1;
-        for (int ix = 0; ix < cx; ++ix)
-            for (int iy = 0; iy < cy; ++iy)
-                for (int iz = 0; iz < cz; ++iz)
-                    if (ix + iy + iz > 0)
-                        ewald_image_f32(F.tcell, fac, F.max_m2, ix ? -nx0 : nx0,
-                                        iy ? -ny0 : ny0, iz ? -kz : kz, a, d);
+// A and dA of the point kx of the line (A, dA as ewald_point_impl, in FP32)
+__device__ __forceinline__ void xline_point(const XLineF32 &L, const FrameDesc &F, int kx,
+                                            float &A, float *dA) {
+    const int nx = kx <= L.half0 ? kx : kx - L.K0;
+    const bool xny = L.even0 && kx == L.half0;
+    const int nimg = (xny ? 2 : 1) * (L.yny ? 2 : 1) * (L.zn2 ? 2 : 1);
+    const float X0 = (float)nx * L.iL00, X1 = (float)nx * L.iL10, X2 = (float)nx * L.iL20;
+    float a = 0.0f, d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, d4 = 0.0f, d5 = 0.0f;
+    for (int k = 0; k < nimg; ++k) {
+        int kk = k;
+        int sx = 1, sy = 1, sz = 1;
+        if (xny) {
+            sx = (kk & 1) ? -1 : 1;
+            kk >>= 1;
+        }
+        if (L.yny) {
+            sy = (kk & 1) ? -1 : 1;
+            kk >>= 1;
+        }
+        if (L.zn2) sz = (kk & 1) ? -1 : 1;
+        const float fx = (float)sx, fy = (float)sy, fz = (float)sz;
+        const float mx = fx * X0;
+        const float my = fx * X1 + fy * L.Y1;
+        const float mz = fx * X2 + fy * L.Y2 + fz * L.Z;
+        const float m2 = mx * mx + my * my + mz * mz;
+        bool in = m2 >= 1e-10f && m2 <= L.max_m2f;
+        if (fabsf(m2 - L.max_m2f) <= 1e-6f * L.max_m2f) {
+#pragma clang fp contract(off)
+            const CellArgs &c = F.tcell;
+            const double nxd = sx * nx, nyd = sy * L.ny, nzd = sz * L.nz;
+            const double mxd = nxd * c.iL00;
+            const double myd = nxd * c.iL10 + nyd * c.iL11;
+            const double mzd = nxd * c.iL20 + nyd * c.iL21 + nzd * c.iL22;
+            const double m2d = mxd * mxd + myd * myd + mzd * mzd;
+            in = !(m2d < 1e-10 || m2d > F.max_m2);
+        }
+        if (in) {
+            const float r = __builtin_amdgcn_rcpf(m2);
+            const float f = __builtin_amdgcn_exp2f(m2 * L.fac2) * r * 0.318309886183790672f;
+            const float fp = (L.fac - r) * f;
+            const float fpx = fp * mx, fpy = fp * my;
+            a += f;
+            d0 += fpx * mx;
+            d1 += fpy * my;
+            d2 += fp * mz * mz;
+            d3 += fpy * mx;
+            d4 += fpx * mz;
+            d5 += fpy * mz;
+        }
     }
-    A = 0.5f * mult * a;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) dA[q] = mult * d[q];
+    A = 0.5f * L.mult * a;
+    dA[0] = L.mult * d0;
+    dA[1] = L.mult * d1;
+    dA[2] = L.mult * d2;
+    dA[3] = L.mult * d3;
+    dA[4] = L.mult * d4;
+    dA[5] = L.mult * d5;
 }
 
 __device__ __forceinline__ void table_point(const FrameDesc &F, long h, int kx, int ky, int kz,
@@ -408,8 +453,12 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         const double two = (kz == 0 || (even && kz == ldim - 1)) ? 2.0 : 1.0;
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
         if (std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
-            // FP32 engine, closed form: FP32 terms, FP64 sums
+            // FP32 engine, closed form: FP32 terms, each thread's sums (at most
+            // kXElems / kBlock points) in FP32, the block sums in FP64
             const float ibyzf = (float)ibyz, twof = (float)two;
+            XLineF32 L;
+            xline_init(L, F, ky, kz);
+            float accf[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 1
             for (int q = 0; q < EPT; ++q) {
                 const int e = ie0 + q * ies;
@@ -419,16 +468,18 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
                     const float re = (float)v.x * ib, im = (float)v.y * ib;  // S-hat
                     const float s = re * re + im * im;
                     float A, dA[6];
-                    ewald_point_f32(F, e, ky, kz, A, dA);
-                    acc[0] += (double)(A * s);
+                    xline_point(L, F, e, A, dA);
+                    accf[0] += A * s;
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) acc[1 + k] += (double)(s * dA[k]);
+                    for (int k = 0; k < 6; ++k) accf[1 + k] += s * dA[k];
                     const float tt = A * ib * twof;
                     v.x = T(re * tt);
                     v.y = T(im * tt);
                     res[cl * LS + e] = v;
                 }
             }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) acc[k] = (double)accf[k];
         } else {
 #pragma unroll 1
             for (int q = 0; q < EPT; ++q) {
