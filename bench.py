@@ -77,6 +77,7 @@ def kernel_bytes(kernel, n, K, p):
         "fft_yz_fwd": G * p + 2 * H * p,       # real plane in, half-complex plane out
         "fft_yz_inv": 2 * H * p + G * p,
         "gather_sorted": G * p + n * (4 * 8 + 4) + n * 4 * 8,
+        "gather_tile": G * p + n * (4 * 8 + 4) + n * 4 * 8,  # halo rows re-read from L2
     }
     return table.get(kernel)
 
@@ -240,6 +241,7 @@ STAGE_OF_KERNEL = {
     "spme::fft_yz_inv_kernel": "fft_yz_inv",
     "spme::fft_z_c2r_kernel": "fft_z_c2r",
     "spme::gather_kernel": "gather_sorted",
+    "spme::gather_tile_kernel": "gather_tile",
 }
 
 
