@@ -113,14 +113,14 @@ class Inputs:
 SET_A_MS = []  # wall time of each influence-table build (device builder + check)
 
 
-def make_handles(systems, precision):
+def make_handles(systems, precision, set_a_ms=None):
     from ewaldcorrel_amd import ewald
     hs = []
     for sy in systems:
         s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
         t0 = time.perf_counter()
         s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
-        SET_A_MS.append(1e3 * (time.perf_counter() - t0))
+        (SET_A_MS if set_a_ms is None else set_a_ms).append(1e3 * (time.perf_counter() - t0))
         hs.append(s)
     return hs
 
@@ -186,7 +186,10 @@ def time_batch(batch, handle0, nframes, steps, warmup, profile):
     return res
 
 
-def run_precision(systems, precision, steps, warmup, profile):
+def run_precision(systems, precision, steps, warmup, profile, tune=True):
+    """Single-frame evaluations with the default launch shapes, then the batch
+    of all frames, autotuned first when `tune` (outside the timed region, as
+    the reference's FFTW planning is)."""
     import torch
     from ewaldcorrel_amd import ewald
     dev = torch.device("cuda:0")
@@ -198,7 +201,14 @@ def run_precision(systems, precision, steps, warmup, profile):
     for f, inp in enumerate(inputs):
         batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
                   inp.dx.data_ptr(), inp.phi.data_ptr())
+    tuned = None
+    if tune:
+        lines = batch.autotune(reps=5).strip().splitlines()
+        tuned = {"report_first": lines[0], "report_last": lines[-1],
+                 "kept": [ln for ln in lines if ln.endswith("(kept)")]}
+        log(f"{precision} autotune: {lines[-1]}; kept {tuned['kept']}")
     bat = time_batch(batch, handles[0], len(handles), steps, warmup, profile)
+    bat["autotune"] = tuned
     bat["table_mode"] = handles[0].info()["table_mode"]
     bat["E"] = [float(inp.out[0].item()) for inp in inputs]
     bat["dx0"] = inputs[0].dx.cpu().numpy()
@@ -206,6 +216,180 @@ def run_precision(systems, precision, steps, warmup, profile):
     batch.close()
     del handles
     return single, bat
+
+
+def uneven_batch(nframes, steps, warmup, tune=True):
+    """256 independent frames of uneven size (atom counts log-uniform in
+    500-20,000, cubic boxes at 0.1 atoms/A^3, K = smallest 2^a 3^b 5^c >= L/1A
+    clamped to [32, 96], order 4; seed 4) evaluated in FP32 as one batch (one
+    launch per stage for all frames, replayed from a captured hipGraph), timed
+    before and after the autotuner chooses the launch shapes of each grid."""
+    import torch
+    from ewaldcorrel_amd import ewald, workloads as wl
+    dev = torch.device("cuda:0")
+    systems = wl.uneven_frames(nframes, seed=4)
+    handles = make_handles(systems, "fp32", [])
+    inputs = [Inputs(sy, dev) for sy in systems]
+    batch = ewald.SfacBatch(handles)
+    for f, inp in enumerate(inputs):
+        batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                  inp.dx.data_ptr(), inp.phi.data_ptr())
+    torch.cuda.synchronize()
+    before = time_batch(batch, handles[0], nframes, steps, warmup, profile=False)
+    E0 = [float(inp.out[0].item()) for inp in inputs]
+    res = {"workload": "uneven_frames", "frames": nframes,
+           "atoms": int(sum(sy.n for sy in systems)),
+           "grid_points": int(sum(np.prod(sy.K) for sy in systems)),
+           "grids": len({sy.K for sy in systems}),
+           "untuned": {"evals_per_s": before["evals_per_s"], "ms_per_batch": before["p50_ms"]}}
+    if tune:
+        t0 = time.perf_counter()
+        rep = batch.autotune(reps=5)
+        lines = rep.strip().splitlines()
+        res["autotune"] = {"seconds": time.perf_counter() - t0, "report_first": lines[0],
+                           "report_last": lines[-1],
+                           "kept": [ln for ln in lines if ln.endswith("(kept)")]}
+    after = time_batch(batch, handles[0], nframes, steps, warmup, profile=False)
+    E1 = [float(inp.out[0].item()) for inp in inputs]
+    Bb = sum(eval_bytes(sy.n, sy.K, 4, 4) for sy in systems)
+    res.update({"evals_per_s": after["evals_per_s"],
+                "latency_ms": {"p50": after["p50_ms"], "p99": after["p99_ms"], "per": "batch"},
+                "eval_roofline_frac": Bb / (after["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "tuned_vs_untuned_energy_rel_max": max(abs(a - b) / abs(b)
+                                                       for a, b in zip(E1, E0))})
+    batch.close()
+    return res
+
+
+def large_box(steps, warmup):
+    """One 1,048,576-charge system (Gaussian charges, 200 x 200 x 400 A,
+    192 x 192 x 384 grid, order 6, beta 0.30; seed 3) evaluated alone, FP32
+    (int64 fixed-point spreading) and FP64."""
+    import torch
+    from ewaldcorrel_amd import workloads as wl
+    dev = torch.device("cuda:0")
+    sy = wl.gaussian_1m()
+    inp = Inputs(sy, dev)
+    res = {"workload": sy.name, "atoms": sy.n, "grid": list(sy.K), "order": sy.order}
+    outs = {}
+    for prec in ("fp32", "fp64"):
+        p = 4 if prec == "fp32" else 8  # grid bytes (the FP32 grid is written as FP32)
+        (h,) = make_handles([sy], prec, [])
+        r = time_single(sy, h, inp, steps, warmup)
+        outs[prec] = (float(inp.out[0].item()), inp.dx.cpu().numpy().copy())
+        res[prec] = {"evals_per_s": r["evals_per_s"],
+                     "latency_ms": {"p50": r["p50_ms"], "p99": r["p99_ms"]},
+                     "eval_roofline_frac": eval_bytes(sy.n, sy.K, p, p) / (r["p50_ms"] * 1e-3)
+                     / 1e9 / HBM_PEAK_GBS}
+        del h
+    (e32, f32), (e64, f64) = outs["fp32"], outs["fp64"]
+    res["fp32_error_vs_fp64_engine"] = {
+        "energy_rel": abs(e32 - e64) / abs(e64),
+        "force_rel_rms": float(np.sqrt(np.mean((f32 - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2)))}
+    return res
+
+
+def serve_stream(count, pool_size, rate, slots, max_batch, seed=5):
+    """The serving stream: `count` requests with Poisson arrivals at `rate`/s
+    submitted to the native serving loop (continuous batching over resident
+    engines), then the same number submitted at once (saturation).
+
+    Requests come from a pool of `pool_size` generated ones (atom counts
+    log-uniform in 1k-500k, triclinic boxes, K_d from the box, order 4 or 6,
+    FP32 or FP64; seed 5) picked at random; each request carries its own box
+    (the pool entry's box scaled by a random factor in [1, 1.001)), so every
+    request builds its influence table on the device, inside its latency.
+    Inputs and outputs are pinned host arrays (the results come back to the
+    host in every request's latency)."""
+    import torch
+    from ewaldcorrel_amd import ewald, workloads as wl
+    rng = np.random.default_rng(seed)
+    pool = [wl.serving_request(rng, 1000, 500000, f"request{i}") for i in range(pool_size)]
+    arrivals = np.cumsum(np.r_[0.0, rng.exponential(1.0 / rate, count - 1)])
+    picks = rng.integers(0, pool_size, 2 * count)
+    scales = 1.0 + 1e-3 * rng.uniform(0, 1, 2 * count)
+
+    def pinned(a):
+        t = torch.empty(a.shape, dtype=torch.float64, pin_memory=True).numpy()
+        t[:] = a
+        return t
+
+    ins = [(pinned(q.w), pinned(q.x)) for q in pool]
+    outs = [(pinned(np.zeros(7)), pinned(np.zeros((q.n, 3))), pinned(np.zeros(q.n)))
+            for q in pool]
+    user = [None] * pool_size  # request currently writing pool entry p's outputs
+    srv = ewald.SfacServer(slots=slots, max_batch=max_batch)
+
+    def submit(p, scale):
+        q = pool[p]
+        if user[p] is not None:
+            srv.wait(user[p])
+        r = srv.submit(q.L6 * scale, q.K, q.order, ins[p][0], ins[p][1], q.beta, q.max_m,
+                       precision=q.meta["precision"], out=outs[p])
+        user[p] = r
+        return r
+
+    def drain():
+        for p in range(pool_size):
+            if user[p] is not None:
+                srv.wait(user[p])
+                user[p] = None
+
+    # warm-up: every pool entry once (engines made resident)
+    for p in range(pool_size):
+        submit(p, 1.0)
+    drain()
+    srv.reset_stats()
+    atoms = 0
+    t0 = time.perf_counter()
+    for i in range(count):
+        t = t0 + arrivals[i]
+        while True:
+            dt = t - time.perf_counter()
+            if dt <= 0:
+                break
+            if dt > 2e-4:
+                time.sleep(dt - 1e-4)
+        submit(int(picks[i]), scales[i])
+        atoms += pool[int(picks[i])].n
+    drain()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    stream = {"offered_req_per_s": rate, "requests": count,
+              "req_per_s": st["completed"] / wall, "atoms_per_s": atoms / wall,
+              "latency_ms": {"p50": 1e3 * st["latency_p50_s"], "p99": 1e3 * st["latency_p99_s"],
+                             "mean": 1e3 * st["latency_mean_s"], "max": 1e3 * st["latency_max_s"]},
+              "queue_wait_ms": {"p50": 1e3 * st["wait_p50_s"], "p99": 1e3 * st["wait_p99_s"]},
+              "failed": st["failed"], "max_running": st["max_running"],
+              "batched_launches": st["batched_launches"],
+              "batched_frames": st["batched_frames"]}
+    # saturation: the same number of requests, all queued at once
+    srv.reset_stats()
+    atoms = 0
+    t0 = time.perf_counter()
+    for i in range(count, 2 * count):
+        submit(int(picks[i]), scales[i])
+        atoms += pool[int(picks[i])].n
+    drain()
+    wall = time.perf_counter() - t0
+    st2 = srv.stats()
+    sat = {"requests": count, "req_per_s": st2["completed"] / wall, "atoms_per_s": atoms / wall,
+           "latency_ms": {"p50": 1e3 * st2["latency_p50_s"], "p99": 1e3 * st2["latency_p99_s"]},
+           "failed": st2["failed"], "max_running": st2["max_running"],
+           "batched_launches": st2["batched_launches"], "batched_frames": st2["batched_frames"]}
+    res = {"workload": "serving_stream", "slots": slots, "max_batch": max_batch,
+           "pool": {"requests": pool_size, "atoms": [min(q.n for q in pool),
+                                                     max(q.n for q in pool)],
+                    "grids": len({q.K for q in pool}),
+                    "fp32": sum(q.meta["precision"] == "fp32" for q in pool),
+                    "order6": sum(q.order == 6 for q in pool)},
+           "resident_engines": st2["resident_engines"],
+           "resident_MB": st2["resident_bytes"] / 1e6, "engines_created": st2["engines_created"],
+           "poisson": stream, "saturated": sat,
+           "latency_includes": "queueing, influence-table build, H2D of x and w, the "
+                               "evaluation, D2H of E, virial, forces and potentials"}
+    srv.close()
+    return res
 
 
 def cpu_baseline(sy, budget_s=15.0):
@@ -289,6 +473,7 @@ def counter_run(systems, steps):
     for f, inp in enumerate(inputs):
         batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
                   inp.dx.data_ptr(), inp.phi.data_ptr())
+    batch.autotune(reps=5)  # the launch shapes the bench measures
     for _ in range(steps):
         batch.eval(None)
     batch.sync()
@@ -312,6 +497,17 @@ def main():
     ap.add_argument("--workload", default="water_98k")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--uneven-frames", type=int, default=256,
+                    help="frames of the uneven batch (0: skip it)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1M-charge box")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the serving stream (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=8)
+    ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="reduce two rocprofv3 counter passes of --counters to the "
                          "per-stage traffic file read by the roofline")
@@ -342,10 +538,12 @@ def main():
     if args.counters:
         counter_run(systems, args.steps)
         return
-    s64, b64 = run_precision(systems, "fp64", args.steps, args.warmup, profile=True)
+    s64, b64 = run_precision(systems, "fp64", args.steps, args.warmup, profile=True,
+                             tune=not args.no_tune)
     log(f"fp64: single {s64['evals_per_s']:.0f} evals/s ({s64['p50_ms']:.3f} ms); "
         f"batch {b64['evals_per_s']:.0f} evals/s, p50 {b64['p50_ms']:.3f} ms/batch")
-    s32, b32 = run_precision(systems, "fp32", args.steps, args.warmup, profile=True)
+    s32, b32 = run_precision(systems, "fp32", args.steps, args.warmup, profile=True,
+                             tune=not args.no_tune)
     log(f"fp32: single {s32['evals_per_s']:.0f} evals/s ({s32['p50_ms']:.3f} ms); "
         f"batch {b32['evals_per_s']:.0f} evals/s, p50 {b32['p50_ms']:.3f} ms/batch")
 
@@ -376,6 +574,27 @@ def main():
     Be32 = eval_bytes(sy.n, sy.K, 4, 4) * B  # the FP32 grid is written once, as FP32
     Be64 = eval_bytes(sy.n, sy.K, 8, 8) * B
 
+    uneven = None
+    if args.uneven_frames > 0:
+        uneven = uneven_batch(args.uneven_frames, max(5, args.steps // 2), args.warmup,
+                              tune=not args.no_tune)
+        log("uneven batch: %.0f evals/s tuned (%.0f untuned), p50 %.3f ms/batch" % (
+            uneven["evals_per_s"], uneven["untuned"]["evals_per_s"],
+            uneven["latency_ms"]["p50"]))
+    large = None
+    if not args.no_large:
+        large = large_box(max(5, args.steps // 2), args.warmup)
+        log("large box: fp32 %.0f evals/s, fp64 %.0f evals/s" % (
+            large["fp32"]["evals_per_s"], large["fp64"]["evals_per_s"]))
+
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_stream(args.serve_requests, args.serve_pool, args.serve_rate,
+                               args.serve_slots, args.serve_batch)
+        log("serving: poisson %.0f req/s p50 %.3f ms p99 %.3f ms; saturated %.0f req/s" % (
+            serving["poisson"]["req_per_s"], serving["poisson"]["latency_ms"]["p50"],
+            serving["poisson"]["latency_ms"]["p99"], serving["saturated"]["req_per_s"]))
+
     cpu = None if args.no_cpu else cpu_baseline(sy, args.cpu_budget)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3f} evals/s ({cpu['sample']})")
@@ -403,6 +622,7 @@ def main():
                      "what": "influence table built on the device, checked against "
                              "the closed form; per frame"},
         "latency_ms": {"p50": b32["p50_ms"], "p99": b32["p99_ms"], "per": "batch"},
+        "autotune": {"fp32": b32["autotune"], "fp64": b64["autotune"]},
         "single_frame": {"fp32": {"evals_per_s": s32["evals_per_s"],
                                   "latency_ms": {"p50": s32["p50_ms"], "p99": s32["p99_ms"]}},
                          "fp64": {"evals_per_s": s64["evals_per_s"],
@@ -421,6 +641,9 @@ def main():
         "kernels_fp32_batch_avg_us": {k: round(v["avg_us"], 3) for k, v in ks.items()},
         "kernels_fp64_batch_avg_us": {k: round(v["avg_us"], 3)
                                       for k, v in b64["kernels"].items()},
+        "uneven_batch": uneven,
+        "large_box": large,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

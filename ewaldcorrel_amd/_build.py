@@ -17,7 +17,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libsfac.so")
 
 HIP_SOURCES = ["engine.hip"]
-CXX_SOURCES = ["sfac_api.cpp"]
+CXX_SOURCES = ["sfac_api.cpp", "server.cpp", "tune.cpp"]
 
 ARCH = os.environ.get("EWALD_OFFLOAD_ARCH", "gfx950")
 

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <mutex>
 #include <type_traits>
 
@@ -244,11 +245,13 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     return ok;
 }
 
-// Spreading tiles may use more than the default 64 KB of dynamic LDS: two
-// 1024-thread workgroups of up to kSpreadLds each still share one CU.  The
-// attribute is set once for every instantiation; without it the tiles stay
-// within 64 KB.
+// Spreading and gather tiles may use more than the default 64 KB of dynamic
+// LDS.  The default tile choice stays within kSpreadLds, so that two
+// 1024-thread workgroups share one CU; an autotuner may choose tiles up to
+// kTileLdsMax (one workgroup per CU).  The attribute is set once for every
+// instantiation; without it the tiles stay within 64 KB.
 constexpr size_t kSpreadLds = 76 * 1024;
+constexpr size_t kTileLdsMax = kSlabLds;
 // LDS of a gather tile: (tx + n) x (ty + n) rows of K2 + n - 1 reals
 size_t gather_tile_lds(int tx, int ty, int K2, int n, size_t tsz) {
     return (size_t)(tx + n) * (ty + n) * (K2 + n - 1) * tsz;
@@ -256,21 +259,21 @@ size_t gather_tile_lds(int tx, int ty, int K2, int n, size_t tsz) {
 template <typename T, int ORDER, int MZ> bool gather_attr() {
     return hipFuncSetAttribute((const void *)gather_tile_kernel<T, ORDER, MZ>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kSpreadLds) == hipSuccess;
+                               (int)kTileLdsMax) == hipSuccess;
 }
 template <typename T, int ORDER> bool spread_attr() {
     bool ok = hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kSpreadLds) == hipSuccess &&
+                                  (int)kTileLdsMax) == hipSuccess &&
               gather_attr<T, ORDER, 1>() && gather_attr<T, ORDER, 2>() &&
               gather_attr<T, ORDER, 3>() && gather_attr<T, ORDER, 4>();
     if (!std::is_same<T, float>::value)
         ok = ok && hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kSpreadLds) == hipSuccess;
+                                       (int)kTileLdsMax) == hipSuccess;
     return ok;
 }
-template <typename T> size_t spread_lds_budget() {
+template <typename T> bool tile_attrs_ok() {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
@@ -278,11 +281,18 @@ template <typename T> size_t spread_lds_budget() {
         ok = hipGetDevice(&dev) == hipSuccess &&
              hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
                  hipSuccess &&
-             (size_t)maxsh >= kSpreadLds && spread_attr<T, 4>() && spread_attr<T, 6>() &&
+             (size_t)maxsh >= kTileLdsMax && spread_attr<T, 4>() && spread_attr<T, 6>() &&
              spread_attr<T, 8>() && spread_attr<T, 0>();
         if (!ok) (void)hipGetLastError();
     });
-    return ok ? kSpreadLds : kLdsBudget;
+    return ok;
+}
+// LDS of the default tile choice, and the most any tile may use
+template <typename T> size_t spread_lds_budget() {
+    return tile_attrs_ok<T>() ? kSpreadLds : kLdsBudget;
+}
+template <typename T> size_t spread_lds_max() {
+    return tile_attrs_ok<T>() ? kTileLdsMax : kLdsBudget;
 }
 
 }  // namespace
@@ -474,6 +484,15 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     D.out = result_;
     ensure_atoms(1024);
     HIPC(hipStreamSynchronize(stream_));
+    // launch shapes tuned earlier in this process for this grid, order and
+    // precision (a record that no longer applies is ignored)
+    TuneCfg tc;
+    if (tune_recall(*this, tc)) {
+        try {
+            apply_tune(tc);
+        } catch (const std::invalid_argument &) {
+        }
+    }
 }
 
 Engine::~Engine() {
@@ -671,10 +690,13 @@ size_t Engine::stage_lds(Stage s) const {
         case ST_YF:
         case ST_YI: return lds_fft(fp32, K[1], D.yC, D.plan1.generic, true);
         case ST_XF_SCALE:
-        case ST_XFUSED:
         case ST_XI:
             return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, D.plan0.generic)
                         : xlines_lds_bytes<double>(K[0], D.xC, D.plan0.generic);
+        case ST_XFUSED:  // the block reduction reuses the LDS
+            return std::max(kPartialScratch,
+                            fp32 ? xlines_lds_bytes<float>(K[0], D.xC, D.plan0.generic)
+                                 : xlines_lds_bytes<double>(K[0], D.xC, D.plan0.generic));
         case ST_YZ_FWD:
         case ST_YZ_INV:
             return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
@@ -862,6 +884,308 @@ double Engine::eval_host(int n, const double *w, const double *x, double vir[6],
     if (vir)
         for (int c = 0; c < 6; ++c) vir[c] = out[1 + c];
     return out[0];
+}
+
+void Engine::upload_atoms(int n, const double *w, const double *x, bool pinned) {
+    if (n < 0) throw std::invalid_argument("n < 0");
+    ensure_atoms(n);
+    if (n == 0) return;
+    const size_t nb = (size_t)n * sizeof(double);
+    const double *xs = x, *ws = w;
+    if (!pinned) {
+        HIPC(hipStreamSynchronize(stream_));  // the staging buffer may still be in flight
+        memcpy(pin_, x, 3 * nb);
+        memcpy(pin_ + 3 * (size_t)n, w, nb);
+        xs = pin_;
+        ws = pin_ + 3 * (size_t)n;
+    }
+    HIPC(hipMemcpyAsync(x_d_, xs, 3 * nb, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(w_d_, ws, nb, hipMemcpyHostToDevice, stream_));
+}
+
+void Engine::evaluate_staged(int n, bool want_dx, bool want_phi) {
+    evaluate(n, w_d_, x_d_, result_, want_dx ? dx_d_ : nullptr, want_phi ? phi_d_ : nullptr);
+}
+
+void Engine::download_results(int n, double *out, double *dx, double *phi, hipStream_t st) {
+    if (out) HIPC(hipMemcpyAsync(out, result_, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dx && n)
+        HIPC(hipMemcpyAsync(dx, dx_d_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            st));
+    if (phi && n)
+        HIPC(hipMemcpyAsync(phi, phi_d_, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            st));
+}
+
+unsigned long long Engine::variant() const {
+    unsigned long long v = 0;
+    v |= (unsigned long long)(fp32 ? 1 : 0);
+    v |= (unsigned long long)(fixed ? 1 : 0) << 1;
+    v |= (unsigned long long)(slab() ? 1 : 0) << 2;
+    v |= (unsigned long long)(gtile() ? 1 : 0) << 3;
+    v |= (unsigned long long)(x_path(desc_) & 0xf) << 4;
+    v |= (unsigned long long)(order & 0xff) << 8;
+    v |= (unsigned long long)(slab_ept() & 0xffff) << 16;
+    v |= (unsigned long long)(gmz() & 0xffff) << 32;
+    return v;
+}
+
+// ---- launch shapes for the autotuner -----------------------------------------------------
+namespace {
+bool plain_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8; }
+}  // namespace
+
+TuneCfg Engine::tune_cfg() const {
+    TuneCfg c;
+    // an axis whose plan has a generic radix is not tuned (nrad 0: keep)
+    for (int d = 0; d < 3; ++d) {
+        bool plain = true;
+        for (int r : plan_[d].radix) plain = plain && plain_radix(r);
+        c.nrad[d] = plain ? (int)plan_[d].radix.size() : 0;
+        for (int s = 0; s < c.nrad[d]; ++s) c.radix[d][s] = plan_[d].radix[s];
+    }
+    c.tx = desc_.tx;
+    c.ty = desc_.ty;
+    c.gtx = desc_.gtx;
+    c.gty = desc_.gty;
+    c.slab = use_slab ? 1 : 0;
+    return c;
+}
+
+namespace {
+bool spread_tile_fits(int tx, int ty, const int K[3], int order, size_t lds_max) {
+    if (tx < 1 || ty < 1 || tx > K[0] || ty > K[1]) return false;
+    if (std::min(tx + order - 1, K[0]) > kMaxSeg / 2) return false;  // column segments
+    return (size_t)tx * ty * (K[2] + order - 1) * 8 <= lds_max;
+}
+
+bool gather_tile_fits(int tx, int ty, const int K[3], int order, size_t lds_max) {
+    if (tx < 1 || ty < 1 || tx > K[0] || ty > K[1] || tx > 64) return false;
+    if (tx * ty < std::min(16, K[0] * K[1])) return false;
+    if (K[2] + order - 1 > 64 * kGatherMaxZ) return false;
+    if ((tx + order) * (ty + order) > 64 * (kGatherBlock / 64)) return false;  // staged rows
+    return gather_tile_lds(tx, ty, K[2], order, 4) <= lds_max;
+}
+}  // namespace
+
+void Engine::apply_tune(const TuneCfg &c) {
+    // validate everything before changing anything
+    for (int d = 0; d < 3; ++d) {
+        if (c.nrad[d] == 0) continue;
+        if (c.nrad[d] < 0 || c.nrad[d] > kMaxRadixStages)
+            throw std::invalid_argument("radix stage count");
+        long long prod = 1;
+        for (int s = 0; s < c.nrad[d]; ++s) {
+            if (!plain_radix(c.radix[d][s])) throw std::invalid_argument("radix not in 2,3,4,5,8");
+            prod *= c.radix[d][s];
+        }
+        if (prod != K[d]) throw std::invalid_argument("radices do not multiply to K");
+        for (int r : plan_[d].radix)
+            if (!plain_radix(r)) throw std::invalid_argument("axis has a generic radix");
+    }
+    const size_t lmax = fp32 ? spread_lds_max<float>() : spread_lds_max<double>();
+    if (c.tx || c.ty)
+        if (!spread_tile_fits(c.tx, c.ty, K, order, lmax))
+            throw std::invalid_argument("spreading tile does not fit");
+    if (c.gtx || c.gty)
+        if (!fp32 || !gather_tile_fits(c.gtx, c.gty, K, order, lmax))
+            throw std::invalid_argument("gather tile does not fit");
+    if (c.slab && !slab_ok) throw std::invalid_argument("slab transforms do not apply");
+    FrameDesc &D = desc_;
+    for (int d = 0; d < 3; ++d) {
+        if (c.nrad[d] == 0) continue;
+        plan_[d].radix.assign(c.radix[d], c.radix[d] + c.nrad[d]);
+        const FftPlanArgs a = plan_args(plan_[d]);
+        if (d == 0) D.plan0 = a;
+        else if (d == 1) D.plan1 = a;
+        else D.plan2 = a;
+    }
+    if (c.tx || c.ty) {
+        D.tx = c.tx;
+        D.ty = c.ty;
+        D.ntx = (K[0] + D.tx - 1) / D.tx;
+        D.nty = (K[1] + D.ty - 1) / D.ty;
+    }
+    D.gtx = c.gtx;
+    D.gty = c.gty;
+    D.gntx = c.gtx ? (K[0] + c.gtx - 1) / c.gtx : 0;
+    D.gnty = c.gty ? (K[1] + c.gty - 1) / c.gty : 0;
+    use_slab = c.slab != 0;
+    drop_graph_();
+}
+
+// Every ordered factorisation of K_d into radices 2, 3, 4, 5, 8 with at most
+// one stage more than the default plan (the default first); at most 32.
+std::vector<std::vector<int>> Engine::radix_options(int d) const {
+    const std::vector<int> &def = plan_[d].radix;
+    std::vector<std::vector<int>> out{def};
+    for (int r : def)
+        if (!plain_radix(r)) return out;
+    const int maxlen = std::min<int>((int)def.size() + 1, kMaxRadixStages);
+    std::vector<std::vector<int>> all;
+    std::vector<int> cur;
+    const int rs[5] = {8, 4, 2, 3, 5};
+    std::function<void(int)> rec = [&](int rem) {
+        if (rem == 1) {
+            if (cur != def) all.push_back(cur);
+            return;
+        }
+        if ((int)cur.size() >= maxlen) return;
+        for (int r : rs)
+            if (rem % r == 0) {
+                cur.push_back(r);
+                rec(rem / r);
+                cur.pop_back();
+            }
+    };
+    if (K[d] > 1) rec(K[d]);
+    std::stable_sort(all.begin(), all.end(), [](const std::vector<int> &a,
+                                                const std::vector<int> &b) {
+        return a.size() < b.size();
+    });
+    for (auto &v : all) {
+        if (out.size() >= 32) break;
+        out.push_back(v);
+    }
+    return out;
+}
+
+std::vector<std::pair<int, int>> Engine::spread_tile_options() const {
+    std::vector<std::pair<int, int>> out;
+    const size_t lmax = fp32 ? spread_lds_max<float>() : spread_lds_max<double>();
+    const int sz[] = {1, 2, 4, 8, 16, 32};
+    for (int a : sz)
+        for (int b : sz) {
+            const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
+            if (tx * ty < std::min(16, K[0] * K[1])) continue;
+            if (!spread_tile_fits(tx, ty, K, order, lmax)) continue;
+            if (std::find(out.begin(), out.end(), std::make_pair(tx, ty)) == out.end())
+                out.push_back({tx, ty});
+        }
+    return out;
+}
+
+std::vector<std::pair<int, int>> Engine::gather_tile_options() const {
+    std::vector<std::pair<int, int>> out{{0, 0}};
+    if (!fp32) return out;
+    const size_t lmax = spread_lds_max<float>();
+    const int sz[] = {2, 4, 8, 16, 32};
+    for (int a : sz)
+        for (int b : sz) {
+            const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
+            if (!gather_tile_fits(tx, ty, K, order, lmax)) continue;
+            if (std::find(out.begin(), out.end(), std::make_pair(tx, ty)) == out.end())
+                out.push_back({tx, ty});
+        }
+    return out;
+}
+
+std::string Engine::tune_key() const {
+    return std::to_string(K[0]) + "x" + std::to_string(K[1]) + "x" + std::to_string(K[2]) +
+           "/" + std::to_string(order) + "/" + std::to_string((fp32 ? 1 : 0) | (fixed ? 2 : 0));
+}
+
+// ---- process-wide record of tuned shapes ------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::vector<std::pair<std::string, TuneCfg>> g_tuned;
+
+std::string cfg_text(const TuneCfg &c) {
+    std::string s;
+    for (int d = 0; d < 3; ++d) {
+        s += d ? " " : "";
+        for (int i = 0; i < c.nrad[d]; ++i) s += (i ? "," : "") + std::to_string(c.radix[d][i]);
+        if (c.nrad[d] == 0) s += "-";
+    }
+    s += " " + std::to_string(c.tx) + "x" + std::to_string(c.ty);
+    s += " " + std::to_string(c.gtx) + "x" + std::to_string(c.gty);
+    s += " " + std::to_string(c.slab);
+    return s;
+}
+}  // namespace
+
+void tune_remember(const Engine &e, const TuneCfg &c) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string k = e.tune_key();
+    for (auto &kv : g_tuned)
+        if (kv.first == k) {
+            kv.second = c;
+            return;
+        }
+    g_tuned.push_back({k, c});
+}
+
+bool tune_recall(const Engine &e, TuneCfg &c) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string k = e.tune_key();
+    for (auto &kv : g_tuned)
+        if (kv.first == k) {
+            c = kv.second;
+            return true;
+        }
+    return false;
+}
+
+void tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
+// one line per key: "K0xK1xK2/order/flags r0 r1 r2 TXxTY GTXxGTY slab", radices
+// comma-separated per axis
+std::string tune_export() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string s;
+    for (auto &kv : g_tuned) s += kv.first + " " + cfg_text(kv.second) + "\n";
+    return s;
+}
+
+int tune_import(const std::string &text) {
+    int nread = 0;
+    size_t pos = 0;
+    while (pos < text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        const std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        char key[64], ax[3][128];
+        int tx, ty, gtx, gty, slab;
+        if (sscanf(line.c_str(), "%63s %127s %127s %127s %dx%d %dx%d %d", key, ax[0], ax[1],
+                   ax[2], &tx, &ty, &gtx, &gty, &slab) != 9)
+            continue;
+        TuneCfg c;
+        bool ok = true;
+        for (int d = 0; d < 3 && ok; ++d) {
+            if (strcmp(ax[d], "-") == 0) continue;
+            const char *p = ax[d];
+            while (*p && ok) {
+                char *end = nullptr;
+                const long r = strtol(p, &end, 10);
+                if (end == p || c.nrad[d] >= kMaxRadixStages || !plain_radix((int)r)) ok = false;
+                else c.radix[d][c.nrad[d]++] = (int)r;
+                p = (*end == ',') ? end + 1 : end;
+                if (*end && *end != ',') ok = false;
+            }
+        }
+        if (!ok) continue;
+        c.tx = tx;
+        c.ty = ty;
+        c.gtx = gtx;
+        c.gty = gty;
+        c.slab = slab;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            bool found = false;
+            for (auto &kv : g_tuned)
+                if (kv.first == key) {
+                    kv.second = c;
+                    found = true;
+                }
+            if (!found) g_tuned.push_back({key, c});
+        }
+        ++nread;
+    }
+    return nread;
 }
 
 double *Engine::get_S_host() {
@@ -1073,20 +1397,26 @@ void Engine::profile_read(std::vector<std::string> &names, std::vector<double> &
 }
 
 // ---- batches --------------------------------------------------------------------------------
-Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
-    if (frames_.empty()) throw std::invalid_argument("empty batch");
-    const Engine *e0 = frames_[0];
-    for (const Engine *e : frames_) {
+Batch::Batch(const std::vector<Engine *> &frames) { rebind(frames); }
+
+void Batch::rebind(const std::vector<Engine *> &frames) {
+    if (frames.empty()) throw std::invalid_argument("empty batch");
+    const Engine *e0 = frames[0];
+    for (const Engine *e : frames) {
         if (e->fp32 != e0->fp32 || e->fixed != e0->fixed || e->order != e0->order)
             throw std::invalid_argument(
                 "a batch needs one precision, accumulation mode and spline order");
         if (!e->has_table()) throw UsageError("No convolution kernel has been set.");
     }
+    frames_ = frames;
     const int nf = (int)frames_.size();
     descs_.resize(nf);
     for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
     slab_ = true;
     gtile_ = true;
+    gmz_ = 1;
+    slab_ept_ = 0;
+    xpath_ = 0;
     for (const Engine *e : frames_) {
         slab_ = slab_ && e->slab();
         gtile_ = gtile_ && e->gtile();
@@ -1094,8 +1424,20 @@ Batch::Batch(const std::vector<Engine *> &frames) : frames_(frames) {
         slab_ept_ = std::max(slab_ept_, e->slab_ept());
         xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
     }
-    HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
-    HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
+    if (nf > cap_) {
+        if (d_descs_) HIPC(hipFree(d_descs_));
+        if (d_offs_) HIPC(hipFree(d_offs_));
+        d_descs_ = nullptr;
+        d_offs_ = nullptr;
+        HIPC(hipMalloc((void **)&d_descs_, nf * sizeof(FrameDesc)));
+        HIPC(hipMalloc((void **)&d_offs_, (size_t)ST_COUNT * (nf + 1) * sizeof(int)));
+        cap_ = nf;
+    }
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    dirty_ = true;
 }
 
 Batch::~Batch() {

@@ -57,6 +57,19 @@ struct FftPlan {
     float2 *tw_f = nullptr;
 };
 
+// Launch shapes of one engine that an autotuner may choose: the radix order
+// of each axis's FFT plan, the spreading tile, the gather tile (0 x 0: the
+// per-atom gather) and the transform path (y-z slabs or separate lines).
+// None of them changes the arithmetic of the deterministic modes beyond the
+// FFT's rounding order.
+struct TuneCfg {
+    int nrad[3] = {0, 0, 0};
+    int radix[3][kMaxRadixStages] = {};
+    int tx = 0, ty = 0;
+    int gtx = 0, gty = 0;
+    int slab = 0;
+};
+
 class Engine {
 public:
     Engine(const double L6[6], const int K[3], int order, int flags);
@@ -90,6 +103,35 @@ public:
     double eval_host(int n, const double *w, const double *x, double vir[6], double *dx,
                      double *phi);
     double *get_S_host();  // host mirror of the grid, K0 x K1 x ldim x 2 doubles
+
+    // Asynchronous host <-> device staging of one evaluation through the
+    // engine's own atom buffers, on stream().  Pinned host memory is copied
+    // directly; pageable input goes through the pinned staging buffer.
+    // download_results writes E and the virial (7 doubles) to out and, when
+    // not NULL, dx and phi; the caller synchronises before reading them.
+    void upload_atoms(int n, const double *w, const double *x, bool pinned);
+    void evaluate_staged(int n, bool want_dx, bool want_phi);
+    void download_results(int n, double *out, double *dx, double *phi, hipStream_t st);
+    const double *x_dev() const { return x_d_; }
+    const double *w_dev() const { return w_d_; }
+    double *dx_dev() const { return dx_d_; }
+    double *phi_dev() const { return phi_d_; }
+    size_t atom_capacity() const { return atom_cap_; }
+    // kernel variants an evaluation of this engine launches (precision,
+    // accumulation, order, transform and gather paths, slab/gather template
+    // sizes); a batch of engines with one signature runs each frame through
+    // exactly the kernels it runs alone
+    unsigned long long variant() const;
+
+    // autotuning: the current launch shapes, applying others (validated
+    // against this grid; std::invalid_argument if one does not apply), and
+    // the candidates a tuner tries
+    TuneCfg tune_cfg() const;
+    void apply_tune(const TuneCfg &c);
+    std::vector<std::vector<int>> radix_options(int d) const;
+    std::vector<std::pair<int, int>> spread_tile_options() const;
+    std::vector<std::pair<int, int>> gather_tile_options() const;
+    std::string tune_key() const;  // "K0xK1xK2/order/flags"
 
     // structure-factor accumulation (device arrays, FP64, caller-owned)
     void accum_S2(double *acc_d);                          // acc[H] += |S|^2
@@ -227,12 +269,17 @@ class Batch {
 public:
     explicit Batch(const std::vector<Engine *> &frames);
     ~Batch();
+    // replace the frames, keeping the device descriptor arrays when they are
+    // large enough (no allocation or free for a batch of at most the largest
+    // earlier size)
+    void rebind(const std::vector<Engine *> &frames);
     // per-frame device pointers for the next evaluation
     void set_atoms(int f, int n, const double *w_d, const double *x_d, double *out_d,
                    double *dx_d, double *phi_d);
     // all frames, one launch per stage; st == nullptr: frame 0's stream
     void evaluate(hipStream_t st);
     int size() const { return (int)frames_.size(); }
+    const std::vector<Engine *> &frames() const { return frames_; }
     hipStream_t default_stream() const { return frames_[0]->stream(); }
     bool use_graph = true;
 
@@ -242,6 +289,7 @@ private:
     std::vector<FrameDesc> descs_;
     FrameDesc *d_descs_ = nullptr;
     int *d_offs_ = nullptr;                 // ST_COUNT x (nf + 1)
+    int cap_ = 0;                           // frames the device arrays hold
     std::vector<int> offs_;                 // host copy
     std::vector<int> total_;                // blocks per stage
     std::vector<size_t> lds_;               // max LDS per stage
@@ -254,6 +302,22 @@ private:
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };
+
+// Process-wide record of tuned launch shapes, keyed by grid, order and
+// precision flags: engines made later with the same key start from it.
+void tune_remember(const Engine &e, const TuneCfg &c);
+bool tune_recall(const Engine &e, TuneCfg &c);
+std::string tune_export();              // one line per key
+int tune_import(const std::string &s);  // entries read (invalid lines skipped)
+void tune_clear();
+
+// Times candidate launch shapes of the frames of `b` (evaluations of the
+// whole batch on `st`, median of `reps` after warm-up), one knob at a time
+// for each distinct (grid, order, flags) class of frames, keeps the fastest
+// (a change must beat the current shape by 1 %), applies it to every frame
+// of the class and remembers it.  Returns a report (one line per class and
+// knob tried).
+std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t st, int reps);
 
 // thrown for API misuse; carries the reference's message
 struct UsageError : std::runtime_error {

@@ -191,20 +191,27 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-reduce acc[7] into one partial per workgroup: F.partials[b] = sums.
-__device__ __forceinline__ void block_partial(const FrameDesc &F, int b, const double *acc) {
-    __shared__ double red[kBlock / 64][8];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < 7; ++c) {
-        const double v = wave_sum(acc[c]);
-        if (lane == 0) red[wv][c] = v;
-    }
+// Block-reduce acc[7] of the kBlock threads into one partial per workgroup,
+// F.partials[b] = sums, in a fixed order: the values go to LDS (`scratch`,
+// kPartialScratch bytes that no thread reads any more), then 32 threads per
+// component add 8 consecutive values each and a 32-lane shuffle tree adds
+// those.  Every thread of the workgroup must call it.
+constexpr size_t kPartialScratch = 7 * kBlock * sizeof(double);
+__device__ __forceinline__ void block_partial(const FrameDesc &F, int b, const double *acc,
+                                              double *scratch) {
     __syncthreads();
-    if (threadIdx.x < 7) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) scratch[c * kBlock + threadIdx.x] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 7 * 32) {
+        const int c = threadIdx.x >> 5, j = threadIdx.x & 31;
+        const double *src = scratch + c * kBlock + j * (kBlock / 32);
         double v = 0.0;
-        for (int q = 0; q < kBlock / 64; ++q) v += red[q][threadIdx.x];
-        F.partials[b * 8 + threadIdx.x] = v;
+#pragma unroll
+        for (int k = 0; k < kBlock / 32; ++k) v += src[k];
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+        if (j == 0) F.partials[b * 8 + c] = v;
     }
 }
 
@@ -281,7 +288,8 @@ __global__ void __launch_bounds__(kBlock) influence_kernel(SPME_FRAME_ARGS) {
             fq[h] = v;
         }
     }
-    block_partial(F, b, acc);
+    __shared__ double scratch[7 * kBlock];
+    block_partial(F, b, acc, scratch);
 }
 
 // ---- y lines (used when the y-z slab does not fit in LDS) --------------------------------
@@ -510,7 +518,7 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
             const int e = e0 + q * es;
             if (e < N && vs) col[(long)e * P] = res[c * LS + e];
         }
-        block_partial(F, b, acc);
+        block_partial(F, b, acc, reinterpret_cast<double *>(smem_raw));  // LDS >= kPartialScratch
         return;
     }
     __syncthreads();

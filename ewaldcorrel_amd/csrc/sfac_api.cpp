@@ -8,12 +8,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "ball.hpp"
 #include "engine.hpp"
+#include "server.hpp"
 #include "sfac.hpp"
 
 namespace {
@@ -407,6 +409,152 @@ int sfac_batch_set_graph(void *b, int enable) {
 
 int sfac_batch_destroy(void *b) {
     return guard([&] { delete reinterpret_cast<spme::Batch *>(b); });
+}
+
+// ---- serving loop --------------------------------------------------------------------------
+void *sfac_server_create(int slots, double hbm_budget, int max_batch, int batch_atoms) {
+    spme::Server *s = nullptr;
+    int rc = guard([&] { s = new spme::Server(slots, hbm_budget, max_batch, batch_atoms); });
+    return rc == 0 ? reinterpret_cast<void *>(s) : nullptr;
+}
+
+long long sfac_server_submit(void *s, const double L6[6], const int K[3], int order, int flags,
+                             double eta, double max_m, int n, const double *w, const double *x,
+                             double *out, double *dx, double *phi) {
+    long long id = 0;
+    int rc = guard([&] {
+        if (!s) throw std::invalid_argument("NULL server");
+        if (!L6 || !K) throw std::invalid_argument("L6 and K are required");
+        id = reinterpret_cast<spme::Server *>(s)->submit(L6, K, order, flags, eta, max_m, n, w,
+                                                         x, out, dx, phi);
+    });
+    return rc == 0 ? id : (long long)rc;
+}
+
+int sfac_server_wait(void *s, long long id, double timing[3]) {
+    int st = 0;
+    int rc = guard([&] {
+        if (!s) throw std::invalid_argument("NULL server");
+        std::string err;
+        st = reinterpret_cast<spme::Server *>(s)->wait(id, timing, &err);
+        if (st != 0) set_error(err);
+    });
+    return rc != 0 ? rc : st;
+}
+
+int sfac_server_poll(void *s, long long id) {
+    int st = 0;
+    int rc = guard([&] {
+        if (!s) throw std::invalid_argument("NULL server");
+        st = reinterpret_cast<spme::Server *>(s)->poll(id);
+    });
+    return rc != 0 ? rc : st;
+}
+
+int sfac_server_stats(void *s, double out[16]) {
+    return guard([&] {
+        if (!s || !out) throw std::invalid_argument("NULL argument");
+        reinterpret_cast<spme::Server *>(s)->stats(out);
+    });
+}
+
+int sfac_server_reset_stats(void *s) {
+    return guard([&] {
+        if (!s) throw std::invalid_argument("NULL server");
+        reinterpret_cast<spme::Server *>(s)->reset_stats();
+    });
+}
+
+int sfac_server_destroy(void *s) {
+    return guard([&] { delete reinterpret_cast<spme::Server *>(s); });
+}
+
+// ---- autotuner -----------------------------------------------------------------------------
+namespace {
+int put_text(const std::string &t, char *buf, int len) {
+    if (buf && len > 0) {
+        const size_t k = std::min<size_t>(t.size(), (size_t)len - 1);
+        memcpy(buf, t.data(), k);
+        buf[k] = 0;
+    }
+    return (int)t.size() + 1;
+}
+}  // namespace
+
+int sfac_autotune(void *h, int n, const double *w_d, const double *x_d, int reps, char *report,
+                  int report_len) {
+    std::string rep;
+    int rc = guard([&] {
+        SFac *s = as_sfac(h);
+        spme::Engine *e = s->engine;
+        if (!e->has_table()) throw spme::UsageError("No convolution kernel has been set.");
+        if (n < 1) throw std::invalid_argument("n < 1");
+        if ((w_d == nullptr) != (x_d == nullptr))
+            throw std::invalid_argument("give both w_d and x_d, or neither");
+        if (!w_d) {
+            // synthetic atoms: uniform in the box, alternating unit charges
+            std::vector<double> w(n), x(3 * (size_t)n);
+            unsigned long long r = 0x9e3779b97f4a7c15ull;
+            auto uni = [&r]() {
+                r ^= r << 13;
+                r ^= r >> 7;
+                r ^= r << 17;
+                return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+            };
+            const double *L = e->L6;
+            for (int a = 0; a < n; ++a) {
+                const double s0 = uni(), s1 = uni(), s2 = uni();
+                x[3 * a + 0] = s0 * L[0] + s1 * L[3] + s2 * L[4];
+                x[3 * a + 1] = s1 * L[1] + s2 * L[5];
+                x[3 * a + 2] = s2 * L[2];
+                w[a] = (a & 1) ? -1.0 : 1.0;
+            }
+            e->upload_atoms(n, w.data(), x.data(), false);
+            e->sync();
+            w_d = e->w_dev();
+            x_d = e->x_dev();
+        }
+        spme::Batch b(std::vector<spme::Engine *>{e});
+        b.set_atoms(0, n, w_d, x_d, e->result_dev(), e->dx_dev(), e->phi_dev());
+        rep = spme::autotune(b.frames(), b, e->stream(), reps);
+        e->sync();
+        e->state = -1;
+    });
+    if (rc != 0) return rc;
+    put_text(rep, report, report_len);
+    return 0;
+}
+
+int sfac_batch_autotune(void *bp, int reps, char *report, int report_len) {
+    std::string rep;
+    int rc = guard([&] {
+        if (!bp) throw std::invalid_argument("NULL batch");
+        spme::Batch *b = reinterpret_cast<spme::Batch *>(bp);
+        rep = spme::autotune(b->frames(), *b, b->default_stream(), reps);
+        spme::hip_check(hipStreamSynchronize(b->default_stream()), "hipStreamSynchronize");
+    });
+    if (rc != 0) return rc;
+    put_text(rep, report, report_len);
+    return 0;
+}
+
+int sfac_tune_export(char *buf, int len) {
+    int need = 0;
+    int rc = guard([&] { need = put_text(spme::tune_export(), buf, len); });
+    return rc != 0 ? rc : need;
+}
+
+int sfac_tune_import(const char *text) {
+    int k = 0;
+    int rc = guard([&] {
+        if (!text) throw std::invalid_argument("NULL text");
+        k = spme::tune_import(text);
+    });
+    return rc != 0 ? rc : k;
+}
+
+int sfac_tune_clear(void) {
+    return guard([&] { spme::tune_clear(); });
 }
 
 }  // extern "C"

@@ -1,0 +1,146 @@
+// Autotuner of launch shapes: FFT radix order per axis, transform path,
+// spreading tile and gather tile, timed on the device over whole batched
+// evaluations (declared in engine.hpp).
+#include <stdio.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace spme {
+
+namespace {
+
+std::string radix_text(const std::vector<int> &r) {
+    std::string s;
+    for (size_t i = 0; i < r.size(); ++i) s += (i ? "," : "") + std::to_string(r[i]);
+    return s;
+}
+
+struct Timer {
+    Batch &b;
+    const std::vector<Engine *> &frames;
+    hipStream_t st;
+    int reps;
+    std::vector<hipEvent_t> ev;
+
+    Timer(Batch &b_, const std::vector<Engine *> &f, hipStream_t s, int r)
+        : b(b_), frames(f), st(s), reps(r), ev(2 * (size_t)r) {
+        for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    }
+    ~Timer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    // median milliseconds of one evaluation of the whole batch
+    double operator()() {
+        b.rebind(frames);  // the frames' current descriptors
+        for (int w = 0; w < 3; ++w) b.evaluate(st);
+        for (int r = 0; r < reps; ++r) {
+            hip_check(hipEventRecord(ev[2 * r], st), "hipEventRecord");
+            b.evaluate(st);
+            hip_check(hipEventRecord(ev[2 * r + 1], st), "hipEventRecord");
+        }
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        std::vector<double> t(reps);
+        for (int r = 0; r < reps; ++r) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]), "hipEventElapsedTime");
+            t[r] = ms;
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    }
+};
+
+}  // namespace
+
+std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t st, int reps) {
+    if (frames.empty()) throw std::invalid_argument("no frames to tune");
+    if (reps < 1) reps = 1;
+    Timer time(b, frames, st, reps);
+    std::string rep;
+    char line[512];
+    double best = time();
+    const double t_start = best;
+    snprintf(line, sizeof(line), "start %.4f ms per batch evaluation (%d frames)\n", best,
+             (int)frames.size());
+    rep += line;
+
+    // classes of frames sharing grid, order and precision
+    std::vector<std::string> keys;
+    for (Engine *e : frames)
+        if (std::find(keys.begin(), keys.end(), e->tune_key()) == keys.end())
+            keys.push_back(e->tune_key());
+
+    for (const std::string &key : keys) {
+        std::vector<Engine *> cls;
+        for (Engine *e : frames)
+            if (e->tune_key() == key) cls.push_back(e);
+        Engine *e0 = cls[0];
+        TuneCfg cur = e0->tune_cfg();
+        for (Engine *e : cls) e->apply_tune(cur);  // one shape for the whole class
+        // try one candidate: keep it if it beats the best by 1 %
+        auto trial = [&](const TuneCfg &c, const std::string &what) {
+            try {
+                for (Engine *e : cls) e->apply_tune(c);
+            } catch (const std::invalid_argument &) {
+                for (Engine *e : cls) e->apply_tune(cur);
+                return;
+            }
+            const double t = time();
+            const bool keep = t < 0.99 * best;
+            snprintf(line, sizeof(line), "%s %s: %.4f ms%s\n", key.c_str(), what.c_str(), t,
+                     keep ? " (kept)" : "");
+            rep += line;
+            if (keep) {
+                best = t;
+                cur = c;
+            } else {
+                for (Engine *e : cls) e->apply_tune(cur);
+            }
+        };
+        // transform path
+        if (e0->slab_ok) {
+            TuneCfg c = cur;
+            c.slab = 1 - cur.slab;
+            trial(c, std::string("transforms ") + (c.slab ? "slab" : "lines"));
+        }
+        // radix order of each axis (z, y, x: the slab passes first)
+        for (int d : {2, 1, 0}) {
+            const auto opts = e0->radix_options(d);
+            for (size_t i = 1; i < opts.size(); ++i) {
+                TuneCfg c = cur;
+                c.nrad[d] = (int)opts[i].size();
+                for (size_t s = 0; s < opts[i].size(); ++s) c.radix[d][s] = opts[i][s];
+                trial(c, "axis " + std::to_string(d) + " radices " + radix_text(opts[i]));
+            }
+        }
+        // spreading tile
+        for (const auto &t : e0->spread_tile_options()) {
+            if (t.first == cur.tx && t.second == cur.ty) continue;
+            TuneCfg c = cur;
+            c.tx = t.first;
+            c.ty = t.second;
+            trial(c, "spread tile " + std::to_string(t.first) + "x" + std::to_string(t.second));
+        }
+        // gather tile (0x0: per-atom gather)
+        for (const auto &t : e0->gather_tile_options()) {
+            if (t.first == cur.gtx && t.second == cur.gty) continue;
+            TuneCfg c = cur;
+            c.gtx = t.first;
+            c.gty = t.second;
+            trial(c, "gather tile " + std::to_string(t.first) + "x" + std::to_string(t.second));
+        }
+        for (Engine *e : cls) e->apply_tune(cur);
+        tune_remember(*e0, cur);
+    }
+    b.rebind(frames);
+    snprintf(line, sizeof(line), "end %.4f ms per batch evaluation (start %.4f)\n", best,
+             t_start);
+    rep += line;
+    return rep;
+}
+
+}  // namespace spme

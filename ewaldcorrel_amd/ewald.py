@@ -23,7 +23,7 @@ from functools import reduce
 from math import pi, exp, erfc  # noqa: F401  (reference module namespace)
 
 from ctypes import CDLL, c_void_p, c_int, c_double, c_char_p, c_longlong, \
-    POINTER, PYFUNCTYPE, cast
+    POINTER, PYFUNCTYPE, cast, create_string_buffer
 
 import numpy as np
 import numpy.ctypeslib as ct
@@ -123,6 +123,19 @@ def describe_sfac():
         int_fn(lib.sfac_batch_sync, c_void_p)
         int_fn(lib.sfac_batch_set_graph, c_void_p, c_int)
         int_fn(lib.sfac_batch_destroy, c_void_p)
+        decl_fn(lib.sfac_server_create, c_int, c_double, c_int, c_int, c_void_p)
+        decl_fn(lib.sfac_server_submit, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
+                c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong)
+        int_fn(lib.sfac_server_wait, c_void_p, c_longlong, c_void_p)
+        int_fn(lib.sfac_server_poll, c_void_p, c_longlong)
+        int_fn(lib.sfac_server_stats, c_void_p, c_void_p)
+        int_fn(lib.sfac_server_reset_stats, c_void_p)
+        int_fn(lib.sfac_server_destroy, c_void_p)
+        int_fn(lib.sfac_autotune, sfac_p, c_int, c_void_p, c_void_p, c_int, c_char_p, c_int)
+        int_fn(lib.sfac_batch_autotune, c_void_p, c_int, c_char_p, c_int)
+        int_fn(lib.sfac_tune_export, c_char_p, c_int)
+        int_fn(lib.sfac_tune_import, c_char_p)
+        int_fn(lib.sfac_tune_clear)
         int_fn(lib.sfac_profile, sfac_p, c_int)
         int_fn(lib.sfac_profile_read, sfac_p, c_char_p, c_int, POINTER(c_double),
                POINTER(c_int), c_int)
@@ -159,6 +172,7 @@ class Sfac:
             flags |= SFAC_FIXED
         self.precision = precision
         L = np.ascontiguousarray(np.asarray(L, dtype=np.float64))
+        self._L6 = L.copy()
         self._data = libsfac.sfac_ctor_ex(L, np.ascontiguousarray(K.astype(np.int32)),
                                           order, flags)
         if not self._data:
@@ -208,6 +222,7 @@ class Sfac:
 
     def set_L(self, L):
         self._libsfac.set_L(self._data, np.ascontiguousarray(L, dtype=np.float64))
+        self._L6 = np.array(L, dtype=np.float64)
         if hasattr(self, "f"):
             self.set_A(self.f, 1.0)
 
@@ -320,6 +335,19 @@ class Sfac:
                                                  float(rc), out_ptr, dx_ptr, phi_ptr),
                "real_space_dev")
 
+    def autotune(self, n=None, w_ptr=None, x_ptr=None, reps=10):
+        """Time and choose launch shapes (transform path, FFT radix orders,
+        spreading and gather tiles) for this handle; returns the report.
+        Without pointers, n synthetic atoms (default: 0.1 per cubic A of the
+        box).  The grid is overwritten (call sfac again afterwards)."""
+        if n is None:
+            Lm = LofS(np.asarray(self._L6))
+            n = max(1000, int(0.1 * abs(np.linalg.det(Lm))))
+        buf = create_string_buffer(1 << 16)
+        _check(self._libsfac.sfac_autotune(self._data, int(n), w_ptr, x_ptr, int(reps), buf,
+                                           len(buf)), "sfac_autotune")
+        return buf.value.decode()
+
     def set_transform(self, mode):
         """0: y-z slab transforms where available (default); 1: z and y line passes."""
         _check(self._libsfac.sfac_set_transform(self._data, int(mode)), "set_transform")
@@ -362,10 +390,135 @@ class SfacBatch:
     def set_graph(self, enable=True):
         _check(self._lib.sfac_batch_set_graph(self._b, int(bool(enable))), "batch set_graph")
 
+    def autotune(self, reps=10):
+        """Time and choose launch shapes for the frames (after set() of every
+        frame); returns the report."""
+        buf = create_string_buffer(1 << 16)
+        _check(self._lib.sfac_batch_autotune(self._b, int(reps), buf, len(buf)),
+               "sfac_batch_autotune")
+        return buf.value.decode()
+
     def close(self):
         if getattr(self, "_b", None):
             self._lib.sfac_batch_destroy(self._b)
             self._b = None
+
+    def __del__(self):
+        self.close()
+
+
+def tune_export():
+    """Launch shapes remembered by the autotuner in this process (text)."""
+    _, _, lib = describe_sfac()
+    need = lib.sfac_tune_export(None, 0)
+    buf = create_string_buffer(max(need, 1))
+    lib.sfac_tune_export(buf, len(buf))
+    return buf.value.decode()
+
+
+def tune_import(text):
+    """Add remembered launch shapes from tune_export() text; returns lines read."""
+    _, _, lib = describe_sfac()
+    k = lib.sfac_tune_import(text.encode())
+    if k < 0:
+        raise RuntimeError("sfac_tune_import failed: " + last_error())
+    return k
+
+
+def tune_clear():
+    _, _, lib = describe_sfac()
+    _check(lib.sfac_tune_clear(), "sfac_tune_clear")
+
+
+class ServeRequest:
+    """One submitted evaluation: holds the borrowed input arrays and the
+    output arrays until ``SfacServer.wait`` returns them."""
+
+    def __init__(self, rid, w, x, out, dx, phi):
+        self.id = rid
+        self._keep = (w, x)
+        self.out, self.dx, self.phi = out, dx, phi
+        self.timing = None
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class SfacServer:
+    """Continuous-batching serving loop over resident engines (native, in
+    libsfac.so): requests of mixed atom counts, boxes, grids, orders and
+    precisions are admitted by free worker slot and HBM budget, overlap on
+    the device, and report their latency.  Each request is one evaluation
+    (E, virial, dE/dx, phi) with the closed-form Ewald kernel of width
+    ``eta`` and cutoff ``max_m``, its influence table built on the device."""
+
+    _STATS = ("completed", "failed", "latency_p50_s", "latency_p99_s", "latency_mean_s",
+              "latency_max_s", "wait_p50_s", "wait_p99_s", "resident_engines",
+              "resident_bytes", "budget_bytes", "engines_created", "evictions",
+              "batched_launches", "batched_frames", "max_running")
+
+    def __init__(self, slots=4, hbm_budget=0, max_batch=8, batch_atoms=32768):
+        _, _, lib = describe_sfac()
+        self._lib = lib
+        self._s = lib.sfac_server_create(int(slots), float(hbm_budget), int(max_batch),
+                                         int(batch_atoms))
+        if not self._s:
+            raise RuntimeError("sfac_server_create failed: " + last_error())
+
+    def submit(self, L, K, order, w, x, eta, max_m=1.0, precision="fp64", fixed=False,
+               forces=True, potentials=True, out=None):
+        """Queue one evaluation.  ``w`` (n) and ``x`` (n x 3) are float64 and
+        stay referenced by the returned request; pinned host arrays are
+        copied by DMA.  ``out`` = optional (E/virial[7], dx[n,3], phi[n])
+        arrays to fill."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = len(w)
+        if x.shape != (n, 3):
+            raise ValueError("x must have shape (n, 3)")
+        flags = (SFAC_FP32 if precision == "fp32" else 0) | (SFAC_FIXED if fixed else 0)
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if out is None:
+            o = np.zeros(7)
+            dx = np.zeros((n, 3)) if forces else None
+            phi = np.zeros(n) if potentials else None
+        else:
+            o, dx, phi = out
+        L = np.ascontiguousarray(np.asarray(L, dtype=np.float64))
+        Ki = np.ascontiguousarray(np.asarray(K, dtype=np.int32))
+        rid = self._lib.sfac_server_submit(self._s, L.ctypes.data, Ki.ctypes.data, int(order),
+                                           flags, float(eta), float(max_m), n, _ptr(w),
+                                           _ptr(x), _ptr(o), _ptr(dx), _ptr(phi))
+        if rid <= 0:
+            raise RuntimeError("sfac_server_submit failed (%d): %s" % (rid, last_error()))
+        return ServeRequest(rid, w, x, o, dx, phi)
+
+    def wait(self, req):
+        """Block until ``req`` ends; returns (E, vir, dx, phi)."""
+        t = np.zeros(3)
+        rc = self._lib.sfac_server_wait(self._s, req.id, t.ctypes.data)
+        req.timing = {"wait_s": t[0], "service_s": t[1], "latency_s": t[2]}
+        if rc != 0:
+            raise RuntimeError("request %d failed (%d): %s" % (req.id, rc, last_error()))
+        return req.out[0], req.out[1:7].copy(), req.dx, req.phi
+
+    def poll(self, req):
+        return self._lib.sfac_server_poll(self._s, req.id)
+
+    def stats(self):
+        out = np.zeros(16)
+        _check(self._lib.sfac_server_stats(self._s, out.ctypes.data), "sfac_server_stats")
+        return dict(zip(self._STATS, out.tolist()))
+
+    def reset_stats(self):
+        _check(self._lib.sfac_server_reset_stats(self._s), "sfac_server_reset_stats")
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.sfac_server_destroy(self._s)
+            self._s = None
 
     def __del__(self):
         self.close()

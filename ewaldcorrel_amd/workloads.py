@@ -8,6 +8,10 @@ reference's ewald.py:122 / test.py:86; Ewald kernel with eta = beta).
                      200 x 200 x 400 A, 192 x 192 x 384, order 6, beta 0.30
   uneven_frames      256 frames, N log-uniform in [500, 20000], density 0.1/A^3,
                      K = smallest 2^a 3^b 5^c >= L/1A clamped to [32, 96], order 4
+  serving_stream     requests arriving as a Poisson process (2,000/s): N log-uniform
+                     in [1000, 500000], triclinic boxes (angles uniform in 70-110
+                     deg) at density 0.1/A^3, K_d = smallest 2^a 3^b 5^c >= |L_d|/1A,
+                     order 4 or 6 and FP32 or FP64 with probability 1/2 each
 """
 from __future__ import annotations
 
@@ -122,3 +126,41 @@ def triclinic_L6(a, b, c, alpha, beta_, gamma):
     if cz2 <= 0:
         raise ValueError("invalid cell angles")
     return np.array([a, by, np.sqrt(cz2), bx, cx, cy])
+
+
+def serving_request(rng, nmin: int = 1000, nmax: int = 500000, name: str = "request"):
+    """One request of the serving stream: N log-uniform in [nmin, nmax]
+    (even, +-1 alternating charges), a triclinic box with all three vectors
+    of length a and angles uniform in [70, 110] deg (resampled until valid)
+    holding N atoms at 0.1 / A^3, K_d = smallest 2^a 3^b 5^c >= a / 1 A,
+    order 4 or 6, FP32 or FP64; eta = 0.30, max_m = 1.0."""
+    n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+    n += n % 2
+    while True:
+        al, be, ga = rng.uniform(70.0, 110.0, 3)
+        ca, cb, cg = cos(radians(al)), cos(radians(be)), cos(radians(ga))
+        vf = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg
+        if vf > 1e-3:
+            break
+    a = (n / 0.1 / np.sqrt(vf)) ** (1 / 3)
+    L6 = triclinic_L6(a, a, a, al, be, ga)
+    k = smooth_235(int(np.ceil(a)))
+    s = rng.uniform(0, 1, (n, 3))
+    Lm = np.array([[L6[0], 0., 0.], [L6[3], L6[1], 0.], [L6[4], L6[5], L6[2]]])
+    x = s @ Lm  # rows of Lm are the box vectors: r = L^T s
+    w = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    order = 4 if rng.uniform() < 0.5 else 6
+    precision = "fp32" if rng.uniform() < 0.5 else "fp64"
+    return System(name, L6, (k, k, k), order, 0.30, w, x,
+                  meta={"precision": precision, "angles": (al, be, ga)})
+
+
+def serving_stream(count: int, seed: int = 5, rate: float = 2000.0, nmin: int = 1000,
+                   nmax: int = 500000):
+    """`count` requests of the serving stream and their Poisson arrival
+    times (seconds from the first, exponential gaps of mean 1/rate)."""
+    rng = np.random.default_rng(seed)
+    reqs = [serving_request(rng, nmin, nmax, f"request{i}") for i in range(count)]
+    gaps = rng.exponential(1.0 / rate, count)
+    gaps[0] = 0.0
+    return reqs, np.cumsum(gaps)

@@ -129,6 +129,75 @@ int sfac_real_space(void *h, int n, const double *w, const double *x, double eta
 int sfac_real_space_dev(void *h, int n, const double *w_d, const double *x_d, double eta,
                         double rc, double *out_d, double *dx_d, double *phi_d);
 
+/* Serving loop: continuous batching of independent evaluations (each like
+ * sfac + de1 + de2 + potl after set_A with the closed-form Ewald kernel, on
+ * its own box, grid, order and precision) on one GPU.  No counterpart in the
+ * reference, whose callers evaluate one system at a time
+ * (run.py:34-45, gmx_ewald.py:76-89).
+ *   sfac_server_create: `slots` worker threads, each with its own HIP
+ *     streams, admit queued requests as soon as they are free and the
+ *     request's engine fits `hbm_budget` bytes of device memory (0: 80 % of
+ *     the free HBM).  Engines (grid, FFT plans, table, sort workspace) stay
+ *     resident per (K, order, flags) and are evicted least recently used.
+ *     A worker that admits a request of at most `batch_atoms` atoms (0:
+ *     32768) also takes up to `max_batch` - 1 more queued small requests
+ *     whose idle engines run the same kernel variants, and evaluates them as
+ *     one batch; each result is the one the request gets alone.
+ *   sfac_server_submit: queue one request (flags: SFAC_FP32 or SFAC_FIXED as
+ *     for sfac_ctor_ex; eta, max_m: the Ewald kernel and cutoff of
+ *     sfac_set_A_ewald).  Host buffers are borrowed until sfac_server_wait
+ *     returns: w[n], x[3n] read; out[7] = E and the virial
+ *     (xx,yy,zz,yx,zx,zy), dx[3n] = dE/dx and phi[n] written (dx, phi may
+ *     be NULL).  Pinned host memory is copied by DMA directly.  Returns the
+ *     request id (> 0) or a negative error code.
+ *   sfac_server_wait: block until the request ends and release it; timing
+ *     (may be NULL) = queue wait, service and total seconds.  Returns 0 or
+ *     the request's error code (message in sfac_last_error).
+ *   sfac_server_poll: 1 done, 0 pending, < 0 failed.
+ *   sfac_server_stats: out[16] = completed, failed, latency p50, p99, mean,
+ *     max (s), queue wait p50, p99 (s), resident engines, HBM bytes they
+ *     hold, budget, engines created, evictions, batched launches, frames in
+ *     them, most requests running at once (since create or reset). */
+void *sfac_server_create(int slots, double hbm_budget, int max_batch, int batch_atoms);
+long long sfac_server_submit(void *s, const double L6[6], const int K[3], int order, int flags,
+                             double eta, double max_m, int n, const double *w, const double *x,
+                             double *out, double *dx, double *phi);
+int sfac_server_wait(void *s, long long id, double timing[3]);
+int sfac_server_poll(void *s, long long id);
+int sfac_server_stats(void *s, double out[16]);
+int sfac_server_reset_stats(void *s);
+int sfac_server_destroy(void *s);
+
+/* Autotuner of launch shapes.  No counterpart in the reference, whose FFT
+ * plans are chosen by FFTW_PATIENT planning in the constructor
+ * (sfac.cpp:78-81); this tunes the transform path (y-z slabs or line
+ * passes), the radix order of each axis's FFT, the spreading tile and the
+ * gather tile by timing whole evaluations on the device, one knob at a time,
+ * keeping a change only if it is 1 % faster.  The result applies to the
+ * handle(s) and is remembered for the process: handles (and serving engines)
+ * made later with the same grid, order and flags start from it.  Outputs of
+ * the deterministic modes are unchanged by the spreading and gather tiles;
+ * a radix order changes only the FFT's rounding.
+ *   sfac_autotune: times single-handle evaluations of n atoms at device
+ *     pointers w_d, x_d (both NULL: n synthetic atoms uniform in the box,
+ *     alternating unit charges); the influence function must be set.  The
+ *     grid is overwritten: call sfac again before en/de1 (state reset).
+ *   sfac_batch_autotune: times evaluations of the whole batch (after
+ *     sfac_batch_set of every frame); frames with the same grid, order and
+ *     flags share one shape.
+ *   Both write a report (one line per candidate) into report[report_len]
+ *   when not NULL; `reps` timed evaluations per candidate (median).
+ *   sfac_tune_export: the remembered shapes as text (one line per key);
+ *     returns the bytes needed including the terminating NUL.
+ *   sfac_tune_import: adds/replaces remembered shapes from such text;
+ *     returns the number of lines read.  sfac_tune_clear forgets them. */
+int sfac_autotune(void *h, int n, const double *w_d, const double *x_d, int reps, char *report,
+                  int report_len);
+int sfac_batch_autotune(void *b, int reps, char *report, int report_len);
+int sfac_tune_export(char *buf, int len);
+int sfac_tune_import(const char *text);
+int sfac_tune_clear(void);
+
 /* Per-kernel timing: while enabled, every launch on the handle's stream is
  * bracketed by HIP events.  sfac_profile_read synchronises, writes up to
  * `max` entries (kernel names joined by '\n' into `names`, summed

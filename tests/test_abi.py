@@ -100,3 +100,32 @@ def test_rad_fn_callback_type_is_accepted(lib):
             return 0.0
 
         assert l.set_A(None, rad_fn(f), 1.0, cast(0, c_void_p)) != 0
+
+
+def test_server_create_fails_cleanly_without_a_device():
+    """With no GPU the serving loop's constructor reports an error instead of
+    aborting (the HIP runtime finds no device)."""
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from ewaldcorrel_amd import ewald
+    with pytest.raises(RuntimeError, match="sfac_server_create failed"):
+        ewald.SfacServer(slots=2)
+
+
+def test_tune_records_round_trip_without_device():
+    """Remembered launch shapes export and import as text (no device call);
+    malformed lines are skipped."""
+    from ewaldcorrel_amd import ewald
+    ewald.tune_clear()
+    text = ("128x128x128/4/3 8,8,2 2,8,8 8,4,4 8x8 8x8 1\n"
+            "45x45x45/6/0 5,3,3 - 3,5,3 4x4 0x0 0\n"
+            "garbage line\n"
+            "32x32x32/4/1 8,4,7 - - 8x8 0x0 0\n")
+    assert ewald.tune_import(text) == 2
+    out = ewald.tune_export()
+    assert "128x128x128/4/3 8,8,2 2,8,8 8,4,4 8x8 8x8 1" in out
+    assert "45x45x45/6/0 5,3,3 - 3,5,3 4x4 0x0 0" in out
+    assert "32x32x32" not in out
+    ewald.tune_clear()
+    assert ewald.tune_export() == ""
