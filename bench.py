@@ -186,7 +186,44 @@ def time_batch(batch, handle0, nframes, steps, warmup, profile):
     return res
 
 
-def run_precision(systems, precision, steps, warmup, profile, tune=True):
+def time_streams(batches, streams, nframes, steps, warmup):
+    """One step = one evaluation of every sub-batch, each on its own HIP
+    stream, so that kernels of different sub-batches overlap.  The step's
+    latency runs from an event on the first stream (which every other stream
+    waits for) to the point where the first stream has waited for all."""
+    import torch
+    S = [torch.cuda.ExternalStream(s) for s in streams]
+    mk = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def step(k, evs):
+        if evs is not None:
+            evs[k][0].record(S[0])
+            for st in S[1:]:
+                st.wait_event(evs[k][0])
+        for b in batches:
+            b.eval(None)
+        if evs is not None:
+            for i, st in enumerate(S[1:]):
+                evs[k][2 + i].record(st)
+                S[0].wait_event(evs[k][2 + i])
+            evs[k][1].record(S[0])
+
+    for k in range(warmup):
+        step(k, None)
+    torch.cuda.synchronize()
+    evs = [[mk() for _ in range(1 + len(S))] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(k, evs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([e[0].elapsed_time(e[1]) for e in evs])
+    return dict(evals_per_s=steps * nframes / wall, ms_per_step=wall * 1e3 / steps,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_event_ms=float(lat.mean()))
+
+
+def run_precision(systems, precision, steps, warmup, profile, tune=True, split=(2, 4)):
     """Single-frame evaluations with the default launch shapes, then the batch
     of all frames, autotuned first when `tune` (outside the timed region, as
     the reference's FFTW planning is)."""
@@ -209,6 +246,27 @@ def run_precision(systems, precision, steps, warmup, profile, tune=True):
         log(f"{precision} autotune: {lines[-1]}; kept {tuned['kept']}")
     bat = time_batch(batch, handles[0], len(handles), steps, warmup, profile)
     bat["autotune"] = tuned
+    # the same frames as S sub-batches on S streams
+    bat["streams"] = {}
+    nf = len(handles)
+    for S in split:
+        if S > nf or nf % S:
+            continue
+        per = nf // S
+        subs = []
+        for g in range(S):
+            sb = ewald.SfacBatch(handles[g * per:(g + 1) * per])
+            for f in range(per):
+                inp = inputs[g * per + f]
+                sb.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                       inp.dx.data_ptr(), inp.phi.data_ptr())
+            subs.append(sb)
+        r = time_streams(subs, [handles[g * per].stream for g in range(S)], nf, steps, warmup)
+        bat["streams"][str(S)] = r
+        log(f"{precision}: {S} streams x {per} frames: {r['evals_per_s']:.0f} evals/s, "
+            f"p50 {r['p50_ms']:.3f} ms/step")
+        for sb in subs:
+            sb.close()
     bat["table_mode"] = handles[0].info()["table_mode"]
     bat["E"] = [float(inp.out[0].item()) for inp in inputs]
     bat["dx0"] = inputs[0].dx.cpu().numpy()
@@ -261,7 +319,7 @@ def uneven_batch(nframes, steps, warmup, tune=True):
     return res
 
 
-def large_box(steps, warmup):
+def large_box(steps, warmup, tune=True):
     """One 1,048,576-charge system (Gaussian charges, 200 x 200 x 400 A,
     192 x 192 x 384 grid, order 6, beta 0.30; seed 3) evaluated alone, FP32
     (int64 fixed-point spreading) and FP64."""
@@ -275,12 +333,29 @@ def large_box(steps, warmup):
     for prec in ("fp32", "fp64"):
         p = 4 if prec == "fp32" else 8  # grid bytes (the FP32 grid is written as FP32)
         (h,) = make_handles([sy], prec, [])
+        tuned = None
+        if tune:
+            lines = h.autotune(sy.n, inp.w.data_ptr(), inp.x.data_ptr(), reps=5).strip()
+            lines = lines.splitlines()
+            tuned = {"report_first": lines[0], "report_last": lines[-1],
+                     "kept": [ln for ln in lines if ln.endswith("(kept)")],
+                     "shapes": h.launch_shapes()}
+            log(f"large box {prec} autotune: {lines[-1]}; {tuned['shapes']}")
         r = time_single(sy, h, inp, steps, warmup)
         outs[prec] = (float(inp.out[0].item()), inp.dx.cpu().numpy().copy())
+        h.profile(True)
+        for _ in range(3):
+            h.eval_dev(inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
+                       inp.dx.data_ptr(), inp.phi.data_ptr())
+        prof = h.profile_read()
+        h.profile(False)
         res[prec] = {"evals_per_s": r["evals_per_s"],
                      "latency_ms": {"p50": r["p50_ms"], "p99": r["p99_ms"]},
                      "eval_roofline_frac": eval_bytes(sy.n, sy.K, p, p) / (r["p50_ms"] * 1e-3)
-                     / 1e9 / HBM_PEAK_GBS}
+                     / 1e9 / HBM_PEAK_GBS,
+                     "kernels_avg_us": {k: round(1e3 * v[0] / max(v[1], 1), 3)
+                                        for k, v in prof.items()},
+                     "autotune": tuned}
         del h
     (e32, f32), (e64, f64) = outs["fp32"], outs["fp64"]
     res["fp32_error_vs_fp64_engine"] = {
@@ -583,7 +658,7 @@ def main():
             uneven["latency_ms"]["p50"]))
     large = None
     if not args.no_large:
-        large = large_box(max(5, args.steps // 2), args.warmup)
+        large = large_box(max(5, args.steps // 2), args.warmup, tune=not args.no_tune)
         log("large box: fp32 %.0f evals/s, fp64 %.0f evals/s" % (
             large["fp32"]["evals_per_s"], large["fp64"]["evals_per_s"]))
 
@@ -623,6 +698,10 @@ def main():
                              "the closed form; per frame"},
         "latency_ms": {"p50": b32["p50_ms"], "p99": b32["p99_ms"], "per": "batch"},
         "autotune": {"fp32": b32["autotune"], "fp64": b64["autotune"]},
+        "sub_batches_on_streams": {
+            prec: {S: {"evals_per_s": r["evals_per_s"], "p50_ms": r["p50_ms"],
+                       "p99_ms": r["p99_ms"]} for S, r in b["streams"].items()}
+            for prec, b in (("fp32", b32), ("fp64", b64))},
         "single_frame": {"fp32": {"evals_per_s": s32["evals_per_s"],
                                   "latency_ms": {"p50": s32["p50_ms"], "p99": s32["p99_ms"]}},
                          "fp64": {"evals_per_s": s64["evals_per_s"],

@@ -37,8 +37,16 @@ struct CellArgs {
 // Division by a launch-invariant divisor d >= 1 without the integer divide
 // sequence: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
 // m = floor(2^32 (2^s - d) / d) + 1, exact for 0 <= n < 2^31.
+// Division by a launch-invariant d without a divide instruction.
+//   fdiv:   any n < 2^31: (mulhi(n, m) + n) >> s (a quarter-rate 32-bit mulhi)
+//   fdiv15: n < 2^15 (every LDS index of the line transforms): with
+//           k = 15 + ceil(log2 d) and m15 = ceil(2^k / d) < 2^17, n m15 < 2^32
+//           and floor(n m15 / 2^k) = floor(n / d) exactly (the rounding error
+//           n (m15 d - 2^k) / (d 2^k) < 2^15 / 2^k <= 1/d cannot carry
+//           floor(n / d) over an integer); one full-rate 24-bit multiply.
 struct FastDiv {
     unsigned d, m, s;
+    unsigned m15, s15;
 };
 inline FastDiv make_fastdiv(unsigned d) {
     FastDiv f;
@@ -46,10 +54,19 @@ inline FastDiv make_fastdiv(unsigned d) {
     f.s = 0;
     while ((1ull << f.s) < d) ++f.s;
     f.m = (unsigned)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
+    f.s15 = 15 + f.s;
+    f.m15 = d < (1u << 15) ? (unsigned)(((1ull << f.s15) + d - 1) / d) : 0u;
     return f;
 }
 __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
     return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
+}
+__device__ __forceinline__ int fdiv15(int n, const FastDiv &f) {
+    return (int)(__umul24((unsigned)n, f.m15) >> f.s15);
+}
+// product of two non-negative ints below 2^24 (LDS indices): full-rate multiply
+__device__ __forceinline__ int mul24(int a, int b) {
+    return (int)__umul24((unsigned)a, (unsigned)b);
 }
 
 struct FftPlanArgs {
@@ -127,7 +144,9 @@ struct FrameDesc {
     int fixed;
     int fx32;        // FP32 spreading: int32 contributions (|q| < 2^30) into int64 sums
     int tx, ty, ntx, nty;  // spreading tiles
+    int tz, ntz;           // z chunks of a spreading tile (ntz = 1: the whole column)
     int gtx, gty, gntx, gnty;  // gather tiles (gtx = 0: no tiled gather)
+    int gtz, gntz;             // z chunks of a gather tile (gntz = 1: the whole column)
     // FFT line blocking
     int zC;          // complex lines (two real rows each) per z workgroup
     int yC, yT;      // lines per y tile, tiles over ldim
@@ -189,6 +208,26 @@ __host__ __device__ inline void scale_wrap(const CellArgs &c, double x0, double 
     s0 -= floor(s0);
     s1 -= floor(s1);
     s2 -= floor(s2);
+}
+
+// Block-wide stream compaction: the keys >= 0 of the workgroup's threads go to
+// list[0..total) in thread order; returns total.  wsum holds a count per
+// wavefront.  Every thread must call it (two barriers).
+__device__ __forceinline__ int block_compact(int key, int *list, int *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long m = __ballot(key >= 0);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < nw; ++q) {
+        const int c = wsum[q];
+        off += q < w ? c : 0;
+        tot += c;
+    }
+    if (key >= 0) list[off + pre] = key;
+    __syncthreads();
+    return tot;
 }
 
 __host__ __device__ inline int imod(int a, int b) {

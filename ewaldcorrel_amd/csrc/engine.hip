@@ -116,6 +116,17 @@ void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int 
         fft_x_kernel<T, MODE, kXGeneric><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
 }
 
+template <typename T, int ORDER, int MZ>
+void gather_tile_launch(const Launch &L, dim3 g, size_t lds, hipStream_t st,
+                        const SplineArgs<T> &sp) {
+    if (L.gzchunk)
+        gather_tile_kernel<T, ORDER, MZ, true><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs,
+                                                                              L.nf, sp);
+    else
+        gather_tile_kernel<T, ORDER, MZ, false><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs,
+                                                                               L.nf, sp);
+}
+
 template <typename T, int ORDER>
 void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
               hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
@@ -126,11 +137,18 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_SCATTER: scatter_kernel<T><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_SPREAD: {
             const dim3 sb(kSpreadBlock);
-            if constexpr (std::is_same<T, float>::value) {
-                spread_tile_kernel<T, ORDER, true><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
+            auto go = [&](auto fixed_c, auto zch_c) {
+                spread_tile_kernel<T, ORDER, decltype(fixed_c)::value, decltype(zch_c)::value>
+                    <<<g, sb, lds, st>>>(L.frames, L.offs, L.nf, sp);
+            };
+            using yes = std::true_type;
+            using no = std::false_type;
+            if (std::is_same<T, float>::value || fixed) {
+                if (L.zchunk) go(yes{}, yes{});
+                else go(yes{}, no{});
             } else {
-                if (fixed) spread_tile_kernel<T, ORDER, true><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
-                else spread_tile_kernel<T, ORDER, false><<<g, sb, lds, st>>>(L.frames, L.offs, L.nf,sp);
+                if (L.zchunk) go(no{}, yes{});
+                else go(no{}, no{});
             }
             break;
         }
@@ -172,10 +190,10 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             break;
         case ST_GATHER_TILE:
             switch (L.gmz) {
-                case 1: gather_tile_kernel<T, ORDER, 1><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
-                case 2: gather_tile_kernel<T, ORDER, 2><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
-                case 3: gather_tile_kernel<T, ORDER, 3><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
-                default: gather_tile_kernel<T, ORDER, 4><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs, L.nf, sp); break;
+                case 1: gather_tile_launch<T, ORDER, 1>(L, g, lds, st, sp); break;
+                case 2: gather_tile_launch<T, ORDER, 2>(L, g, lds, st, sp); break;
+                case 3: gather_tile_launch<T, ORDER, 3>(L, g, lds, st, sp); break;
+                default: gather_tile_launch<T, ORDER, 4>(L, g, lds, st, sp); break;
             }
             break;
         default: break;
@@ -257,20 +275,30 @@ size_t gather_tile_lds(int tx, int ty, int K2, int n, size_t tsz) {
     return (size_t)(tx + n) * (ty + n) * (K2 + n - 1) * tsz;
 }
 template <typename T, int ORDER, int MZ> bool gather_attr() {
-    return hipFuncSetAttribute((const void *)gather_tile_kernel<T, ORDER, MZ>,
+    return hipFuncSetAttribute((const void *)gather_tile_kernel<T, ORDER, MZ, false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kTileLdsMax) == hipSuccess &&
+           hipFuncSetAttribute((const void *)gather_tile_kernel<T, ORDER, MZ, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kTileLdsMax) == hipSuccess;
 }
 template <typename T, int ORDER> bool spread_attr() {
-    bool ok = hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true>,
+    bool ok = hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kTileLdsMax) == hipSuccess &&
+              hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, true, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)kTileLdsMax) == hipSuccess &&
               gather_attr<T, ORDER, 1>() && gather_attr<T, ORDER, 2>() &&
               gather_attr<T, ORDER, 3>() && gather_attr<T, ORDER, 4>();
     if (!std::is_same<T, float>::value)
-        ok = ok && hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kTileLdsMax) == hipSuccess;
+        ok = ok &&
+             hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kTileLdsMax) == hipSuccess &&
+             hipFuncSetAttribute((const void *)spread_tile_kernel<T, ORDER, false, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kTileLdsMax) == hipSuccess;
     return ok;
 }
 template <typename T> bool tile_attrs_ok() {
@@ -429,16 +457,32 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
     }
-    // spreading tiles: rows of K2 + order - 1 cells (the z stencil unwrapped)
+    // spreading tiles: rows of K2 + order - 1 cells (the z stencil unwrapped);
+    // columns too long for an 8 x 8 or 8 x 4 tile are split into z chunks of an
+    // 8 x 8 tile
     static const int tiles[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
     const size_t spread_budget =
         fp32 ? spread_lds_budget<float>() : spread_lds_budget<double>();
+    D.tz = K[2];
+    D.ntz = 1;
     for (auto &t : tiles) {
         const int tx = std::min(t[0], K[0]), ty = std::min(t[1], K[1]);
         if ((size_t)tx * ty * (K[2] + order - 1) * 8 <= spread_budget) {
             D.tx = tx;
             D.ty = ty;
             break;
+        }
+    }
+    if (D.tx * D.ty < std::min(32, K[0] * K[1]) && K[2] >= 2 * order) {
+        const int tx = std::min(8, K[0]), ty = std::min(8, K[1]);
+        const int zmax =
+            (int)((spread_budget - kSpreadBlock * sizeof(int)) / ((size_t)tx * ty * 8));
+        if (zmax >= order) {
+            const int nch = (K[2] + zmax - 1) / zmax;
+            D.tx = tx;
+            D.ty = ty;
+            D.tz = (K[2] + nch - 1) / nch;
+            D.ntz = (K[2] + D.tz - 1) / D.tz;
         }
     }
     if (D.tx == 0) throw std::invalid_argument("K2 too large for the spreading tile");
@@ -448,6 +492,9 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     // must fit the per-lane row loads, and a tile of fewer than 16 columns stages
     // too much halo per atom (the per-atom gather is used then; FP64 rows double
     // the LDS and register footprint, and the per-atom gather is faster there)
+    // (columns too long for any of these tiles: z chunks of an 8 x 8 tile)
+    D.gtz = K[2];
+    D.gntz = 1;
     if (fp32 && K[2] + order - 1 <= 64 * kGatherMaxZ) {
         static const int gtiles[][2] = {{8, 8}, {8, 4}, {4, 4}};
         for (auto &t : gtiles) {
@@ -460,6 +507,21 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                 D.gnty = (K[1] + ty - 1) / ty;
                 break;
             }
+        }
+    }
+    if (fp32 && D.gtx == 0 && K[2] >= 2 * order) {
+        const int tx = std::min(8, K[0]), ty = std::min(8, K[1]);
+        const int zlds = (int)((spread_budget - kGatherBlock * sizeof(int)) /
+                               ((size_t)(tx + order) * (ty + order) * tsz));
+        const int zmax = std::min(64 * kGatherMaxZ, zlds) - (order - 1);
+        if (tx * ty >= std::min(16, K[0] * K[1]) && zmax >= order) {
+            const int nch = (K[2] + zmax - 1) / zmax;
+            D.gtz = (K[2] + nch - 1) / nch;
+            D.gntz = (K[2] + D.gtz - 1) / D.gtz;
+            D.gtx = tx;
+            D.gty = ty;
+            D.gntx = (K[0] + tx - 1) / tx;
+            D.gnty = (K[1] + ty - 1) / ty;
         }
     }
 
@@ -655,7 +717,7 @@ int Engine::stage_blocks(Stage s) const {
         case ST_PREP:
         case ST_SCATTER: return (D.n + kBlock - 1) / kBlock;
         case ST_SCAN: return 1;
-        case ST_SPREAD: return D.ntx * D.nty;
+        case ST_SPREAD: return D.ntx * D.nty * D.ntz;
         case ST_ZR2C:
         case ST_ZC2R: return (nrows + 2 * D.zC - 1) / (2 * D.zC);
         case ST_YF:
@@ -673,7 +735,7 @@ int Engine::stage_blocks(Stage s) const {
             const int apb = kBlock / order;
             return (D.n + apb - 1) / apb;
         }
-        case ST_GATHER_TILE: return D.gntx * D.gnty;
+        case ST_GATHER_TILE: return D.gntx * D.gnty * D.gntz;
         default: return 0;
     }
 }
@@ -681,9 +743,13 @@ int Engine::stage_blocks(Stage s) const {
 size_t Engine::stage_lds(Stage s) const {
     const FrameDesc &D = desc_;
     switch (s) {
-        case ST_SPREAD: return (size_t)D.tx * D.ty * (K[2] + order - 1) * 8;
-        case ST_GATHER_TILE:
-            return gather_tile_lds(D.gtx, D.gty, K[2], order, fp32 ? 4 : 8);
+        case ST_SPREAD:  // chunked tiles: the compacted atom list after the cells
+            return (size_t)D.tx * D.ty * (D.ntz > 1 ? D.tz : K[2] + order - 1) * 8 +
+                   (D.ntz > 1 ? kSpreadBlock * sizeof(int) : 0);
+        case ST_GATHER_TILE:  // chunked tiles: the compacted atom list after the rows
+            return gather_tile_lds(D.gtx, D.gty, D.gntz > 1 ? D.gtz : K[2], order,
+                                   fp32 ? 4 : 8) +
+                   (D.gntz > 1 ? kGatherBlock * sizeof(int) : 0);
 
         case ST_ZR2C:
         case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);
@@ -728,7 +794,9 @@ void Engine::run_(Stage s, int finish) {
     if (nb <= 0) return;
     sync_desc_();
     ProfScope ps(this, stage_name(s));
-    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz()}, nb, stage_lds(s), stream_,
+    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
+                     desc_.gntz > 1}, nb,
+           stage_lds(s), stream_,
            finish);
 }
 
@@ -927,6 +995,8 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(order & 0xff) << 8;
     v |= (unsigned long long)(slab_ept() & 0xffff) << 16;
     v |= (unsigned long long)(gmz() & 0xffff) << 32;
+    v |= (unsigned long long)(desc_.ntz > 1 ? 1 : 0) << 48;
+    v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
     return v;
 }
 
@@ -946,25 +1016,38 @@ TuneCfg Engine::tune_cfg() const {
     }
     c.tx = desc_.tx;
     c.ty = desc_.ty;
+    c.tz = desc_.ntz > 1 ? desc_.tz : 0;
     c.gtx = desc_.gtx;
     c.gty = desc_.gty;
+    c.gtz = desc_.gntz > 1 ? desc_.gtz : 0;
     c.slab = use_slab ? 1 : 0;
     return c;
 }
 
 namespace {
-bool spread_tile_fits(int tx, int ty, const int K[3], int order, size_t lds_max) {
-    if (tx < 1 || ty < 1 || tx > K[0] || ty > K[1]) return false;
-    if (std::min(tx + order - 1, K[0]) > kMaxSeg / 2) return false;  // column segments
-    return (size_t)tx * ty * (K[2] + order - 1) * 8 <= lds_max;
+// a z chunk (tz > 0) must hold a whole stencil, and the column at least two
+bool z_chunk_ok(int tz, const int K[3], int order) {
+    return tz == 0 || (tz >= order && tz < K[2] && K[2] >= 2 * order);
 }
 
-bool gather_tile_fits(int tx, int ty, const int K[3], int order, size_t lds_max) {
+bool spread_tile_fits(int tx, int ty, int tz, const int K[3], int order, size_t lds_max) {
+    if (tx < 1 || ty < 1 || tx > K[0] || ty > K[1]) return false;
+    if (std::min(tx + order - 1, K[0]) > kMaxSeg / 2) return false;  // column segments
+    if (!z_chunk_ok(tz, K, order)) return false;
+    return (size_t)tx * ty * (tz ? tz : K[2] + order - 1) * 8 +
+               (tz ? kSpreadBlock * sizeof(int) : 0) <= lds_max;
+}
+
+bool gather_tile_fits(int tx, int ty, int tz, const int K[3], int order, size_t lds_max,
+                      size_t tsz) {
     if (tx < 1 || ty < 1 || tx > K[0] || ty > K[1] || tx > 64) return false;
     if (tx * ty < std::min(16, K[0] * K[1])) return false;
-    if (K[2] + order - 1 > 64 * kGatherMaxZ) return false;
+    if (!z_chunk_ok(tz, K, order)) return false;
+    const int zl = tz ? tz : K[2];
+    if (zl + order - 1 > 64 * kGatherMaxZ) return false;
     if ((tx + order) * (ty + order) > 64 * (kGatherBlock / 64)) return false;  // staged rows
-    return gather_tile_lds(tx, ty, K[2], order, 4) <= lds_max;
+    return gather_tile_lds(tx, ty, zl, order, tsz) + (tz ? kGatherBlock * sizeof(int) : 0) <=
+           lds_max;
 }
 }  // namespace
 
@@ -985,10 +1068,10 @@ void Engine::apply_tune(const TuneCfg &c) {
     }
     const size_t lmax = fp32 ? spread_lds_max<float>() : spread_lds_max<double>();
     if (c.tx || c.ty)
-        if (!spread_tile_fits(c.tx, c.ty, K, order, lmax))
+        if (!spread_tile_fits(c.tx, c.ty, c.tz, K, order, lmax))
             throw std::invalid_argument("spreading tile does not fit");
     if (c.gtx || c.gty)
-        if (!fp32 || !gather_tile_fits(c.gtx, c.gty, K, order, lmax))
+        if (!gather_tile_fits(c.gtx, c.gty, c.gtz, K, order, lmax, fp32 ? 4 : 8))
             throw std::invalid_argument("gather tile does not fit");
     if (c.slab && !slab_ok) throw std::invalid_argument("slab transforms do not apply");
     FrameDesc &D = desc_;
@@ -1005,11 +1088,15 @@ void Engine::apply_tune(const TuneCfg &c) {
         D.ty = c.ty;
         D.ntx = (K[0] + D.tx - 1) / D.tx;
         D.nty = (K[1] + D.ty - 1) / D.ty;
+        D.tz = c.tz ? c.tz : K[2];
+        D.ntz = (K[2] + D.tz - 1) / D.tz;
     }
     D.gtx = c.gtx;
     D.gty = c.gty;
     D.gntx = c.gtx ? (K[0] + c.gtx - 1) / c.gtx : 0;
     D.gnty = c.gty ? (K[1] + c.gty - 1) / c.gty : 0;
+    D.gtz = c.gtz ? c.gtz : K[2];
+    D.gntz = (K[2] + D.gtz - 1) / D.gtz;
     use_slab = c.slab != 0;
     drop_graph_();
 }
@@ -1050,33 +1137,36 @@ std::vector<std::vector<int>> Engine::radix_options(int d) const {
     return out;
 }
 
-std::vector<std::pair<int, int>> Engine::spread_tile_options() const {
-    std::vector<std::pair<int, int>> out;
+std::vector<std::array<int, 3>> Engine::spread_tile_options() const {
+    std::vector<std::array<int, 3>> out;
     const size_t lmax = fp32 ? spread_lds_max<float>() : spread_lds_max<double>();
     const int sz[] = {1, 2, 4, 8, 16, 32};
+    const int zs[] = {0, 32, 64, 128, 256};
     for (int a : sz)
-        for (int b : sz) {
-            const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
-            if (tx * ty < std::min(16, K[0] * K[1])) continue;
-            if (!spread_tile_fits(tx, ty, K, order, lmax)) continue;
-            if (std::find(out.begin(), out.end(), std::make_pair(tx, ty)) == out.end())
-                out.push_back({tx, ty});
-        }
+        for (int b : sz)
+            for (int tz : zs) {
+                const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
+                if (tx * ty < std::min(16, K[0] * K[1])) continue;
+                if (!spread_tile_fits(tx, ty, tz, K, order, lmax)) continue;
+                const std::array<int, 3> t{tx, ty, tz};
+                if (std::find(out.begin(), out.end(), t) == out.end()) out.push_back(t);
+            }
     return out;
 }
 
-std::vector<std::pair<int, int>> Engine::gather_tile_options() const {
-    std::vector<std::pair<int, int>> out{{0, 0}};
-    if (!fp32) return out;
-    const size_t lmax = spread_lds_max<float>();
+std::vector<std::array<int, 3>> Engine::gather_tile_options() const {
+    std::vector<std::array<int, 3>> out{{0, 0, 0}};
+    const size_t lmax = fp32 ? spread_lds_max<float>() : spread_lds_max<double>();
     const int sz[] = {2, 4, 8, 16, 32};
+    const int zs[] = {0, 32, 64, 128, 192};
     for (int a : sz)
-        for (int b : sz) {
-            const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
-            if (!gather_tile_fits(tx, ty, K, order, lmax)) continue;
-            if (std::find(out.begin(), out.end(), std::make_pair(tx, ty)) == out.end())
-                out.push_back({tx, ty});
-        }
+        for (int b : sz)
+            for (int tz : zs) {
+                const int tx = std::min(a, K[0]), ty = std::min(b, K[1]);
+                if (!gather_tile_fits(tx, ty, tz, K, order, lmax, fp32 ? 4 : 8)) continue;
+                const std::array<int, 3> t{tx, ty, tz};
+                if (std::find(out.begin(), out.end(), t) == out.end()) out.push_back(t);
+            }
     return out;
 }
 
@@ -1097,8 +1187,9 @@ std::string cfg_text(const TuneCfg &c) {
         for (int i = 0; i < c.nrad[d]; ++i) s += (i ? "," : "") + std::to_string(c.radix[d][i]);
         if (c.nrad[d] == 0) s += "-";
     }
-    s += " " + std::to_string(c.tx) + "x" + std::to_string(c.ty);
-    s += " " + std::to_string(c.gtx) + "x" + std::to_string(c.gty);
+    s += " " + std::to_string(c.tx) + "x" + std::to_string(c.ty) + "x" + std::to_string(c.tz);
+    s += " " + std::to_string(c.gtx) + "x" + std::to_string(c.gty) + "x" +
+         std::to_string(c.gtz);
     s += " " + std::to_string(c.slab);
     return s;
 }
@@ -1126,13 +1217,16 @@ bool tune_recall(const Engine &e, TuneCfg &c) {
     return false;
 }
 
+std::string tune_line(const Engine &e) { return e.tune_key() + " " + cfg_text(e.tune_cfg()); }
+
 void tune_clear() {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned.clear();
 }
 
-// one line per key: "K0xK1xK2/order/flags r0 r1 r2 TXxTY GTXxGTY slab", radices
-// comma-separated per axis
+// one line per key: "K0xK1xK2/order/flags r0 r1 r2 TXxTYxTZ GTXxGTYxGTZ slab",
+// radices comma-separated per axis ("-": the default plan), TZ/GTZ = 0 for
+// whole columns (an "AxB" tile without z is read as whole columns)
 std::string tune_export() {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     std::string s;
@@ -1148,11 +1242,15 @@ int tune_import(const std::string &text) {
         if (eol == std::string::npos) eol = text.size();
         const std::string line = text.substr(pos, eol - pos);
         pos = eol + 1;
-        char key[64], ax[3][128];
-        int tx, ty, gtx, gty, slab;
-        if (sscanf(line.c_str(), "%63s %127s %127s %127s %dx%d %dx%d %d", key, ax[0], ax[1],
-                   ax[2], &tx, &ty, &gtx, &gty, &slab) != 9)
+        char key[64], ax[3][128], st[64], gt[64];
+        int slab;
+        if (sscanf(line.c_str(), "%63s %127s %127s %127s %63s %63s %d", key, ax[0], ax[1],
+                   ax[2], st, gt, &slab) != 7)
             continue;
+        int tx = 0, ty = 0, tz = 0, gtx = 0, gty = 0, gtz = 0;
+        const int ns = sscanf(st, "%dx%dx%d", &tx, &ty, &tz);
+        const int ng = sscanf(gt, "%dx%dx%d", &gtx, &gty, &gtz);
+        if (ns < 2 || ng < 2) continue;
         TuneCfg c;
         bool ok = true;
         for (int d = 0; d < 3 && ok; ++d) {
@@ -1170,8 +1268,10 @@ int tune_import(const std::string &text) {
         if (!ok) continue;
         c.tx = tx;
         c.ty = ty;
+        c.tz = tz;
         c.gtx = gtx;
         c.gty = gty;
+        c.gtz = gtz;
         c.slab = slab;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1417,7 +1517,11 @@ void Batch::rebind(const std::vector<Engine *> &frames) {
     gmz_ = 1;
     slab_ept_ = 0;
     xpath_ = 0;
+    zchunk_ = 0;
+    gzchunk_ = 0;
     for (const Engine *e : frames_) {
+        zchunk_ = zchunk_ || e->desc().ntz > 1;
+        gzchunk_ = gzchunk_ || e->desc().gntz > 1;
         slab_ = slab_ && e->slab();
         gtile_ = gtile_ && e->gtile();
         gmz_ = std::max(gmz_, e->gmz());
@@ -1500,7 +1604,8 @@ void Batch::evaluate(hipStream_t st) {
         const StageSeq seq = eval_stages(slab_, gtile_);
         for (int i = 0; i < seq.n; ++i) {
             const Stage s = seq.s[i];
-            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_};
+            const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_,
+                           zchunk_, gzchunk_};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
             e0->prof_end_on(st);

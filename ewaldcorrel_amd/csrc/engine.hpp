@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,6 +49,8 @@ struct Launch {
     int slab_ept;  // elements per thread the slab kernels are compiled for
     int xpath;     // transform path of the x launches (XPath)
     int gmz;       // row chunks of 64 per lane of the tiled gather (ceil((K2 + n - 1) / 64))
+    int zchunk;    // some frame spreads through z-chunked tiles
+    int gzchunk;   // some frame gathers through z-chunked tiles
 };
 
 struct FftPlan {
@@ -65,8 +68,8 @@ struct FftPlan {
 struct TuneCfg {
     int nrad[3] = {0, 0, 0};
     int radix[3][kMaxRadixStages] = {};
-    int tx = 0, ty = 0;
-    int gtx = 0, gty = 0;
+    int tx = 0, ty = 0, tz = 0;     // spreading tile; tz = 0: whole columns
+    int gtx = 0, gty = 0, gtz = 0;  // gather tile; gtz = 0: whole columns
     int slab = 0;
 };
 
@@ -129,8 +132,9 @@ public:
     TuneCfg tune_cfg() const;
     void apply_tune(const TuneCfg &c);
     std::vector<std::vector<int>> radix_options(int d) const;
-    std::vector<std::pair<int, int>> spread_tile_options() const;
-    std::vector<std::pair<int, int>> gather_tile_options() const;
+    // (tx, ty, tz) candidates; tz = 0: whole columns
+    std::vector<std::array<int, 3>> spread_tile_options() const;
+    std::vector<std::array<int, 3>> gather_tile_options() const;  // includes {0, 0, 0}
     std::string tune_key() const;  // "K0xK1xK2/order/flags"
 
     // structure-factor accumulation (device arrays, FP64, caller-owned)
@@ -160,7 +164,9 @@ public:
     int slab_ept() const;
     // the evaluation gathers through LDS-staged tiles when one fits (FrameDesc::gtx)
     bool gtile() const { return desc_.gtx > 0; }
-    int gmz() const { return (K[2] + order - 1 + 63) / 64; }
+    int gmz() const {
+        return ((desc_.gntz > 1 ? desc_.gtz : K[2]) + order - 1 + 63) / 64;
+    }
 
     // Per-kernel timing with HIP events recorded on stream() around every
     // launch while enabled (graph replay is off meanwhile).
@@ -299,6 +305,8 @@ private:
     int gmz_ = 1;                           // largest row chunk count of the frames
     int slab_ept_ = 0;                      // slab kernel variant (largest frame)
     int xpath_ = 0;                         // x transform path valid for every frame
+    int zchunk_ = 0;                        // some frame spreads through z-chunked tiles
+    int gzchunk_ = 0;                       // some frame gathers through z-chunked tiles
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };
@@ -308,6 +316,7 @@ private:
 void tune_remember(const Engine &e, const TuneCfg &c);
 bool tune_recall(const Engine &e, TuneCfg &c);
 std::string tune_export();              // one line per key
+std::string tune_line(const Engine &e);  // the engine's current shapes, one line
 int tune_import(const std::string &s);  // entries read (invalid lines skipped)
 void tune_clear();
 

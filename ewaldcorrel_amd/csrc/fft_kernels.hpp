@@ -151,14 +151,15 @@ __device__ __forceinline__ void bfly8(cplx<T> *v, int sign) {
 
 // In-place stage of fixed radix R over C lines (line c at buf + c*LS):
 // every thread loads the inputs of up to MAXQ butterflies into registers,
-// transforms them, and after a barrier stores the outputs.
+// transforms them, and after a barrier stores the outputs.  Index arithmetic
+// by FastDivs and 24-bit multiplies (every index is an LDS index < 2^15).
 template <typename T, int R, int EPT = fft_elems<T>() / kBlock>
 __device__ __forceinline__ void stage_ip(cplx<T> *buf, int C, int LS, int N, int Ns,
+                                         const FastDiv &dNR, const FastDiv &dNs, int twstep,
                                          const cplx<T> *tw, int sign) {
     constexpr int MAXQ = (EPT + R - 1) / R;
-    const int NR = N / R;
-    const int nb = C * NR;
-    const int twstep = N / (Ns * R);
+    const int NR = dNR.d;
+    const int nb = C * NR;  // wave-uniform products stay on the scalar unit
     cplx<T> v[MAXQ][R];
     int dst[MAXQ];
 #pragma unroll
@@ -166,23 +167,28 @@ __device__ __forceinline__ void stage_ip(cplx<T> *buf, int C, int LS, int N, int
         const int bf = threadIdx.x + q * kBlock;
         dst[q] = -1;
         if (bf < nb) {
-            const int c = bf / NR;
-            const int j = bf - c * NR;
-            const int k = j % Ns;
-            const cplx<T> *s = buf + c * LS + j;
+            const int c = fdiv15(bf, dNR);
+            const int j = bf - mul24(c, NR);
+            const int jq = fdiv15(j, dNs);
+            const int k = j - mul24(jq, Ns);
+            const cplx<T> *s = buf + mul24(c, LS) + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = s[r * NR];
             if (Ns > 1) {
+                const int kt = mul24(k, twstep);
+                int t = 0;
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[q][r] = cmul<T>(v[q][r], twiddle<T>(tw, r * k * twstep, sign));
+                for (int r = 1; r < R; ++r) {
+                    t += kt;
+                    v[q][r] = cmul<T>(v[q][r], twiddle<T>(tw, t, sign));
+                }
             }
             if (R == 2) bfly2<T>(v[q]);
             else if (R == 3) bfly3<T>(v[q], sign);
             else if (R == 4) bfly4<T>(v[q], sign);
             else if (R == 5) bfly5<T>(v[q], sign);
             else if (R == 8) bfly8<T>(v[q], sign);
-            dst[q] = c * LS + (j / Ns) * Ns * R + k;
+            dst[q] = mul24(c, LS) + mul24(jq, Ns * (R - 1)) + j;
         }
     }
     __syncthreads();
@@ -236,11 +242,11 @@ __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, i
     for (int s = 0; s < p.nstages; ++s) {
         const int R = p.radix[s];
         switch (R) {
-            case 2: stage_ip<T, 2, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 3: stage_ip<T, 3, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 4: stage_ip<T, 4, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 5: stage_ip<T, 5, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
-            case 8: stage_ip<T, 8, EPT>(cur, C, LS, p.N, Ns, tw, sign); break;
+            case 2: stage_ip<T, 2, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
+            case 3: stage_ip<T, 3, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
+            case 4: stage_ip<T, 4, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
+            case 5: stage_ip<T, 5, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
+            case 8: stage_ip<T, 8, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
             default: {
                 stage_generic<T>(cur, oth, C, LS, p.N, Ns, R, tw, sign);
                 cplx<T> *t = cur;

@@ -183,8 +183,9 @@ constexpr int kGatherBlock = 512;
 constexpr int kGatherLoadRegs = 24;     // staged values each lane holds before storing
 constexpr int kGatherMaxZ = 4;          // row length K2 + n - 1 <= 64 kGatherMaxZ
 
-// MZ = ceil((K2 + n - 1) / 64) row chunks per lane
-template <typename T, int ORDER, int MZ>
+// MZ = ceil((K2 + n - 1) / 64) row chunks per lane; ZCH: launches with
+// z-chunked frames
+template <typename T, int ORDER, int MZ, bool ZCH>
 __global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(4)))
 gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -194,10 +195,18 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     const int n = order_of<ORDER>(F.order), h = n / 2;
     constexpr int CAP = Ord<ORDER>::cap;
     const int K0 = F.K0, K1 = F.K1, K2 = F.K2;
-    const int X = b % F.gntx, Y = b / F.gntx;
+    // tile (X, Y) of columns and, for long columns, z chunk Zc of F.gtz cells:
+    // the chunk's atoms (ceil cell k2 in [zc0, zc0 + tzr)) with rows staged
+    // from zc0 - n/2 to zc0 + tzr + n/2 - 2
+    const int nxy = F.gntx * F.gnty;
+    const int Zc = b / nxy, bxy = b - Zc * nxy;
+    const int X = bxy % F.gntx, Y = bxy / F.gntx;
     const int x0 = X * F.gtx, y0 = Y * F.gty;
     const int txr = min(F.gtx, K0 - x0), tyr = min(F.gty, K1 - y0);
-    const int PY = tyr + n, Kp = K2 + n - 1;
+    const bool zchunk = ZCH && F.gntz > 1;
+    const int zc0 = Zc * F.gtz;
+    const int tzr = zchunk ? min(F.gtz, K2 - zc0) : K2;
+    const int PY = tyr + n, Kp = tzr + n - 1;
     const int rows = (txr + n) * PY;
     T *tile = reinterpret_cast<T *>(smem_raw);
 
@@ -239,7 +248,7 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         for (int m = 0; m < MZ; ++m) {
             int zp = lane + 64 * m;
             if (zp >= Kp) zp = m > 0 ? zdst[m - 1] : Kp - 1;
-            int z = zp - h;
+            int z = zc0 + zp - h;
             if (K2 >= n) z = z < 0 ? z + K2 : (z >= K2 ? z - K2 : z);
             else z = imod(z, K2);
             zsrc[m] = z;
@@ -283,11 +292,8 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     const int units = run_pref[64];
     typedef __attribute__((address_space(1))) const AtomRec<T> GRec;
     GRec *recs = (GRec *)F.recs;
-    for (int u = threadIdx.x; u < units; u += blockDim.x) {
-        int lo = 0;
-        for (int d = 1; d < txr; ++d)
-            if (run_pref[d] <= u) lo = d;
-        GRec &r = recs[run_begin[lo] + (u - run_pref[lo])];
+    // the stencil work of one atom (record r)
+    auto atom = [&](GRec &r) {
         const int a = r.perm;
         const double wa = r.w;
         const int cell = r.cell;
@@ -299,6 +305,7 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         const int b0 = (f1 > T(0) ? c1 + 1 : c1) - y0;
         int k2 = f2 > T(0) ? c2 + 1 : c2;
         if (k2 >= K2) k2 -= K2;
+        if (zchunk) k2 -= zc0;  // the compaction kept this chunk's atoms only
         const T t0 = f0 > T(0) ? T(1) - f0 : T(0);
         const T t1 = f1 > T(0) ? T(1) - f1 : T(0);
         const T t2 = f2 > T(0) ? T(1) - f2 : T(0);
@@ -311,7 +318,11 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         int plane = (a0 * PY + b0) * Kp + k2;
         const int step_x = PY * Kp;
         T e0 = 0, e1 = 0, e2 = 0, ph = 0;
-#pragma unroll
+        // planes unrolled for FP32 up to order 4; FP64 and wider stencils keep
+        // one plane's n^2 loads in flight (a full unroll needs far more than
+        // 128 VGPRs)
+        constexpr int kPlanesUnrolled = sizeof(T) == 4 && ORDER > 0 && ORDER <= 4 ? CAP : 1;
+#pragma unroll kPlanesUnrolled
         for (int i = 0; i < CAP; ++i) {
             if (i >= n) break;
             T pb = 0, pdb = 0, pbdz = 0;
@@ -350,6 +361,43 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
             F.dx[3 * a + 2] = cl.iL20 * g0 + cl.iL21 * g1 + cl.iL22 * g2;
         }
         if (F.phi) F.phi[a] = (double)ph * cl.iV;
+    };
+    if constexpr (!ZCH) {
+        // one atom per thread
+        for (int u = threadIdx.x; u < units; u += blockDim.x) {
+            int lo = 0;
+            for (int d = 1; d < txr; ++d)
+                if (run_pref[d] <= u) lo = d;
+            atom(recs[run_begin[lo] + (u - run_pref[lo])]);
+        }
+    } else {
+        // One atom per thread and round; in a z chunk, the round's atoms of the
+        // chunk are first compacted into clist, so that the wavefronts that do
+        // stencil work are full.
+        __shared__ int wsum[kGatherBlock / 64];
+        int *clist = reinterpret_cast<int *>(tile + rows * Kp);  // kGatherBlock ints
+        for (int ub = 0; ub < units; ub += blockDim.x) {
+            int ridx = -1;
+            const int u = ub + (int)threadIdx.x;
+            if (u < units) {
+                int lo = 0;
+                for (int d = 1; d < txr; ++d)
+                    if (run_pref[d] <= u) lo = d;
+                ridx = run_begin[lo] + (u - run_pref[lo]);
+                if (zchunk) {  // ceil cell in z inside the chunk?
+                    const int c2 = recs[ridx].cell & 1023;
+                    int k2 = recs[ridx].f[2] > T(0) ? c2 + 1 : c2;
+                    if (k2 >= K2) k2 -= K2;
+                    if ((unsigned)(k2 - zc0) >= (unsigned)tzr) ridx = -1;
+                }
+            }
+            if (zchunk) {
+                const int cnt = block_compact(ridx, clist, wsum);
+                ridx = (int)threadIdx.x < cnt ? clist[threadIdx.x] : -1;
+            }
+            if (ridx >= 0) atom(recs[ridx]);
+            if (zchunk) __syncthreads();  // clist is rewritten by the next round
+        }
     }
 }
 

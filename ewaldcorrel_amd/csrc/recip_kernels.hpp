@@ -448,7 +448,8 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
     }
     __syncthreads();
     const int s0 = MODE == 0 ? sign : -1;
-    const FastDiv dC = {(unsigned)C, 1u, (unsigned)lg};  // power of two: n div C = n >> lg
+    // power of two: n div C = n >> lg (fdiv: mulhi(n, 1) = 0; fdiv15: n 2^15 >> (15 + lg))
+    const FastDiv dC = {(unsigned)C, 1u, (unsigned)lg, 1u << 15, 15u + (unsigned)lg};
     cplx<T> *res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
     if (MODE == 2) {
         const int cl = c, ie0 = e0, ies = es;

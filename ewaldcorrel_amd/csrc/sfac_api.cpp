@@ -514,6 +514,9 @@ int sfac_autotune(void *h, int n, const double *w_d, const double *x_d, int reps
             w_d = e->w_dev();
             x_d = e->x_dev();
         }
+        // the outputs go to the engine's staging buffers: size them for n first
+        // (growing replaces the buffers, so take their addresses afterwards)
+        e->ensure_atoms(n);
         spme::Batch b(std::vector<spme::Engine *>{e});
         b.set_atoms(0, n, w_d, x_d, e->result_dev(), e->dx_dev(), e->phi_dev());
         rep = spme::autotune(b.frames(), b, e->stream(), reps);
@@ -551,6 +554,12 @@ int sfac_tune_import(const char *text) {
         k = spme::tune_import(text);
     });
     return rc != 0 ? rc : k;
+}
+
+int sfac_tune_current(void *h, char *buf, int len) {
+    int need = 0;
+    int rc = guard([&] { need = put_text(spme::tune_line(*as_sfac(h)->engine), buf, len); });
+    return rc != 0 ? rc : need;
 }
 
 int sfac_tune_clear(void) {

@@ -219,7 +219,7 @@ __device__ __forceinline__ int round_half_up(float x) {
     return q;
 }
 
-template <typename T, int ORDER, bool FIXED>
+template <typename T, int ORDER, bool FIXED, bool ZCH>
 __global__ void __launch_bounds__(kSpreadBlock)
 spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -230,10 +230,18 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     const int n = order_of<ORDER>(F.order);
     constexpr int CAP = Ord<ORDER>::cap;
     const int K0 = F.K0, K1 = F.K1, K2 = F.K2;
-    const int X = b % F.ntx, Y = b / F.ntx;
+    // tile (X, Y) of columns and, for long columns, z chunk Zc of F.tz cells
+    const int nxy = F.ntx * F.nty;
+    const int Zc = b / nxy, bxy = b - Zc * nxy;
+    const int X = bxy % F.ntx, Y = bxy / F.ntx;
     const int x0 = X * F.tx, y0 = Y * F.ty;
     const int txr = min(F.tx, K0 - x0), tyr = min(F.ty, K1 - y0);
-    const int Kp = K2 + n - 1;
+    const bool zchunk = ZCH && F.ntz > 1;  // ZCH: launches with chunked frames
+    const int zc0 = Zc * F.tz;
+    const int tzr = zchunk ? min(F.tz, K2 - zc0) : K2;
+    // row length: the whole column with the z stencil unwrapped (folded back at
+    // write-out), or the chunk's own cells only (stencil points outside dropped)
+    const int Kp = zchunk ? tzr : K2 + n - 1;
     const int cells = txr * tyr * Kp;
 
     typedef typename std::conditional<FIXED, unsigned long long, double>::type Acc;
@@ -285,19 +293,22 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     typedef __attribute__((address_space(1))) const AtomRec<T> GRec;
     GRec *recs = (GRec *)F.recs;
 
-    for (int u = threadIdx.x; u < units; u += blockDim.x) {
-        int lo = 0, hi = nseg - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (seg_pref[mid] <= u) lo = mid;
-            else hi = mid - 1;
-        }
-        const int d = lo >> 1;
-        GRec &r = recs[seg_begin[lo] + (u - seg_pref[lo])];
+    const int H = n / 2 - 1;
+    // the stencil work of one atom (record r, neighbourhood row d)
+    auto atom = [&](GRec &r, int d) {
         const int cell = r.cell;
         const int cx = cell >> 20;
         const int cy = (cell >> 10) & 1023;
         const int cz = cell & 1023;
+        // z chunk: the stencil covers z = cz - H .. cz + n/2 (mod K2), H = n/2 - 1;
+        // zoff = its first cell in chunk coordinates when it does not wrap
+        int zoff = cz;
+        bool zfast = true;
+        if (zchunk) {
+            zoff = cz - H - zc0;
+            zfast = zoff >= 0 && zoff + n - 1 < tzr;
+            if (!zfast) zoff = 0;
+        }
         T bx[CAP], by[CAP], bz[CAP];
         spline_w<T, ORDER>(sp, n, r.f[0], bx);
         spline_w<T, ORDER>(sp, n, r.f[1], by);
@@ -308,20 +319,37 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         typedef typename std::conditional<std::is_same<T, float>::value && FIXED, float,
                                           double>::type W;
         const W ws = W(r.w * scale);
+        auto add = [&](Acc *cellp, W wxy, int k) {
+            if (std::is_same<W, float>::value) {
+                const int qv = round_half_up((float)wxy * (float)bz[k]);
+                atomicAdd((unsigned long long *)cellp, (unsigned long long)(long long)qv);
+            } else if (FIXED) {
+                const long long qv = llrint((double)wxy * (double)bz[k]);
+                atomicAdd((unsigned long long *)cellp, (unsigned long long)qv);
+            } else {
+                atomicAdd((double *)cellp, (double)wxy * (double)bz[k]);
+            }
+        };
         auto line = [&](Acc *row, W wxy) {
-            // row points at zz = cz; stencil point k lands at row[n - 1 - k]
+            if (zfast) {
+                // row points at the stencil's first cell (zz = cz, or the chunk
+                // offset); stencil point k lands at row[n - 1 - k]
 #pragma unroll
-            for (int k = 0; k < CAP; ++k) {
-                if (k >= n) break;
-                if (std::is_same<W, float>::value) {
-                    const int qv = round_half_up((float)wxy * (float)bz[k]);
-                    atomicAdd((unsigned long long *)&row[n - 1 - k],
-                              (unsigned long long)(long long)qv);
-                } else if (FIXED) {
-                    const long long qv = llrint((double)wxy * (double)bz[k]);
-                    atomicAdd((unsigned long long *)&row[n - 1 - k], (unsigned long long)qv);
-                } else {
-                    atomicAdd((double *)&row[n - 1 - k], (double)wxy * (double)bz[k]);
+                for (int k = 0; k < CAP; ++k) {
+                    if (k >= n) break;
+                    add(&row[n - 1 - k], wxy, k);
+                }
+            } else {
+                // chunk edge or wrap: row points at the chunk's first cell; only
+                // the stencil points inside the chunk are added
+#pragma unroll
+                for (int k = 0; k < CAP; ++k) {
+                    if (k >= n) break;
+                    int z = cz + n / 2 - k;
+                    if (z >= K2) z -= K2;
+                    else if (z < 0) z += K2;
+                    const int zl = z - zc0;
+                    if ((unsigned)zl < (unsigned)tzr) add(&row[zl], wxy, k);
                 }
             }
         };
@@ -333,7 +361,7 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
             else if (dy >= K1) dy -= K1;
             // the line of plane i, row j starts (i tyr + j) Kp cells before the
             // line of plane 0, row 0: a workgroup-uniform offset
-            Acc *base = acc + ((d * tyr + dy) * Kp + cz);
+            Acc *base = acc + ((d * tyr + dy) * Kp + zoff);
             const int sx = tyr * Kp;
 #pragma unroll
             for (int i = 0; i < CAP; ++i) {
@@ -361,9 +389,51 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
                     if (j >= n) break;
                     const int ly = imod(cy + n / 2 - j, K1) - y0;
                     if ((unsigned)ly >= (unsigned)tyr) continue;
-                    line(acc + ((lx * tyr + ly) * Kp + cz), wx * W(by[j]));
+                    line(acc + ((lx * tyr + ly) * Kp + zoff), wx * W(by[j]));
                 }
             }
+        }
+    };
+    if constexpr (!ZCH) {
+        // one atom per thread
+        for (int u = threadIdx.x; u < units; u += blockDim.x) {
+            int lo = 0, hi = nseg - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (seg_pref[mid] <= u) lo = mid;
+                else hi = mid - 1;
+            }
+            atom(recs[seg_begin[lo] + (u - seg_pref[lo])], lo >> 1);
+        }
+    } else {
+        // One atom per thread and round; in a z chunk, the round's atoms whose
+        // stencil reaches the chunk are first compacted into clist, so that
+        // the wavefronts that do stencil work are full.
+        __shared__ int wsum[kSpreadBlock / 64];
+        int *clist = reinterpret_cast<int *>(acc + cells);  // kSpreadBlock ints
+        for (int ub = 0; ub < units; ub += blockDim.x) {
+            int key = -1;  // record index << 5 | neighbourhood row d
+            const int u = ub + (int)threadIdx.x;
+            if (u < units) {
+                int lo = 0, hi = nseg - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (seg_pref[mid] <= u) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int ridx = seg_begin[lo] + (u - seg_pref[lo]);
+                key = (ridx << 5) | (lo >> 1);
+                if (zchunk) {  // the stencil z = cz - H .. cz + n/2 (mod K2) misses the chunk?
+                    const int l0 = imod((recs[ridx].cell & 1023) - H - zc0, K2);
+                    if (l0 >= tzr && l0 + n - 1 < K2) key = -1;
+                }
+            }
+            if (zchunk) {
+                const int cnt = block_compact(key, clist, wsum);
+                key = (int)threadIdx.x < cnt ? clist[threadIdx.x] : -1;
+            }
+            if (key >= 0) atom(recs[key >> 5], key & 31);
+            if (zchunk) __syncthreads();  // clist is rewritten by the next round
         }
     }
     __syncthreads();
@@ -374,13 +444,24 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     const double inv = FIXED ? 1.0 / scale : 1.0;
     const long rs = 2L * F.ldim;
     const int rows = txr * tyr;
-    const int H = n / 2 - 1;
     const int lane = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
     for (int rr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); rr < rows; rr += nw) {
         const int lx = rr / tyr, ly = rr - lx * tyr;
         const Acc *arow = acc + rr * Kp;
         GT *grow = grid + ((long)(x0 + lx) * K1 + (y0 + ly)) * rs;
+        if (zchunk) {  // the chunk's own cells, nothing to fold
+            for (int zl0 = 0; zl0 < tzr; zl0 += 64) {
+                const int zl = zl0 + lane;
+                if (zl >= tzr) continue;
+                const Acc v = arow[zl];
+                double o;
+                if (FIXED) o = (double)(long long)v * inv;
+                else o = (double)v;
+                grow[zc0 + zl] = T(o);
+            }
+            continue;
+        }
         for (int z0 = 0; z0 < K2; z0 += 64) {
             const int z = z0 + lane;
             if (z >= K2) continue;

@@ -4,6 +4,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -117,21 +118,27 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
                 trial(c, "axis " + std::to_string(d) + " radices " + radix_text(opts[i]));
             }
         }
-        // spreading tile
+        auto tile_text = [](const std::array<int, 3> &t) {
+            return std::to_string(t[0]) + "x" + std::to_string(t[1]) +
+                   (t[2] ? "x" + std::to_string(t[2]) : std::string());
+        };
+        // spreading tile (z chunks of tz cells, or whole columns)
         for (const auto &t : e0->spread_tile_options()) {
-            if (t.first == cur.tx && t.second == cur.ty) continue;
+            if (t[0] == cur.tx && t[1] == cur.ty && t[2] == cur.tz) continue;
             TuneCfg c = cur;
-            c.tx = t.first;
-            c.ty = t.second;
-            trial(c, "spread tile " + std::to_string(t.first) + "x" + std::to_string(t.second));
+            c.tx = t[0];
+            c.ty = t[1];
+            c.tz = t[2];
+            trial(c, "spread tile " + tile_text(t));
         }
         // gather tile (0x0: per-atom gather)
         for (const auto &t : e0->gather_tile_options()) {
-            if (t.first == cur.gtx && t.second == cur.gty) continue;
+            if (t[0] == cur.gtx && t[1] == cur.gty && t[2] == cur.gtz) continue;
             TuneCfg c = cur;
-            c.gtx = t.first;
-            c.gty = t.second;
-            trial(c, "gather tile " + std::to_string(t.first) + "x" + std::to_string(t.second));
+            c.gtx = t[0];
+            c.gty = t[1];
+            c.gtz = t[2];
+            trial(c, "gather tile " + tile_text(t));
         }
         for (Engine *e : cls) e->apply_tune(cur);
         tune_remember(*e0, cur);

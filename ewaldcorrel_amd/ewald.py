@@ -136,6 +136,7 @@ def describe_sfac():
         int_fn(lib.sfac_tune_export, c_char_p, c_int)
         int_fn(lib.sfac_tune_import, c_char_p)
         int_fn(lib.sfac_tune_clear)
+        int_fn(lib.sfac_tune_current, sfac_p, c_char_p, c_int)
         int_fn(lib.sfac_profile, sfac_p, c_int)
         int_fn(lib.sfac_profile_read, sfac_p, c_char_p, c_int, POINTER(c_double),
                POINTER(c_int), c_int)
@@ -346,6 +347,14 @@ class Sfac:
         buf = create_string_buffer(1 << 16)
         _check(self._libsfac.sfac_autotune(self._data, int(n), w_ptr, x_ptr, int(reps), buf,
                                            len(buf)), "sfac_autotune")
+        return buf.value.decode()
+
+    def launch_shapes(self):
+        """The handle's current launch shapes as a tune_export() line."""
+        buf = create_string_buffer(512)
+        rc = self._libsfac.sfac_tune_current(self._data, buf, len(buf))
+        if rc < 0:
+            raise RuntimeError("sfac_tune_current failed: " + last_error())
         return buf.value.decode()
 
     def set_transform(self, mode):

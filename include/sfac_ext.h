@@ -190,12 +190,19 @@ int sfac_server_destroy(void *s);
  *   sfac_tune_export: the remembered shapes as text (one line per key);
  *     returns the bytes needed including the terminating NUL.
  *   sfac_tune_import: adds/replaces remembered shapes from such text;
- *     returns the number of lines read.  sfac_tune_clear forgets them. */
+ *     returns the number of lines read.  sfac_tune_clear forgets them.
+ *   sfac_tune_current: the handle's current shapes as one such line
+ *     (returns the bytes needed including the NUL).
+ * Line format: "K0xK1xK2/order/flags R0 R1 R2 TXxTYxTZ GTXxGTYxGTZ slab" with
+ * each axis's radices comma-separated ("-": the default plan), TZ and GTZ
+ * the z chunk of the spreading and gather tiles (0: whole columns), a gather
+ * tile 0x0x0 for the per-atom gather, slab 1 for the y-z slab transforms. */
 int sfac_autotune(void *h, int n, const double *w_d, const double *x_d, int reps, char *report,
                   int report_len);
 int sfac_batch_autotune(void *b, int reps, char *report, int report_len);
 int sfac_tune_export(char *buf, int len);
 int sfac_tune_import(const char *text);
+int sfac_tune_current(void *h, char *buf, int len);
 int sfac_tune_clear(void);
 
 /* Per-kernel timing: while enabled, every launch on the handle's stream is

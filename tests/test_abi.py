@@ -119,13 +119,15 @@ def test_tune_records_round_trip_without_device():
     from ewaldcorrel_amd import ewald
     ewald.tune_clear()
     text = ("128x128x128/4/3 8,8,2 2,8,8 8,4,4 8x8 8x8 1\n"
-            "45x45x45/6/0 5,3,3 - 3,5,3 4x4 0x0 0\n"
+            "45x45x45/6/0 5,3,3 - 3,5,3 4x4x0 0x0x0 0\n"
+            "192x192x384/6/3 - - - 8x8x128 8x8x96 0\n"
             "garbage line\n"
             "32x32x32/4/1 8,4,7 - - 8x8 0x0 0\n")
-    assert ewald.tune_import(text) == 2
+    assert ewald.tune_import(text) == 3
     out = ewald.tune_export()
-    assert "128x128x128/4/3 8,8,2 2,8,8 8,4,4 8x8 8x8 1" in out
-    assert "45x45x45/6/0 5,3,3 - 3,5,3 4x4 0x0 0" in out
+    assert "128x128x128/4/3 8,8,2 2,8,8 8,4,4 8x8x0 8x8x0 1" in out
+    assert "45x45x45/6/0 5,3,3 - 3,5,3 4x4x0 0x0x0 0" in out
+    assert "192x192x384/6/3 - - - 8x8x128 8x8x96 0" in out
     assert "32x32x32" not in out
     ewald.tune_clear()
     assert ewald.tune_export() == ""

@@ -107,3 +107,23 @@ def test_batch_autotune_on_uneven_frames():
         assert rel_rms(dx, before[f][1]) <= 1e-5, f
     b.close()
     ew().tune_clear()
+
+
+def test_autotune_on_caller_device_atoms_larger_than_the_initial_buffers():
+    """Tuning on the caller's device arrays with more atoms than a new
+    handle's staging buffers hold: the buffers are sized for n before the
+    tuner's evaluations write forces and potentials into them."""
+    ew().tune_clear()
+    L6, K, order, beta, w, x = system([40., 40., 40., 0., 0., 0.], (40, 40, 40), 6, 20000, 8)
+    s = ew().Sfac(L6, np.array(K), order, precision="fp32")
+    s.set_A(ew().ewald_kernel(beta), 1.0)
+    a = s.evaluate(w, x)
+    dev = torch.device("cuda:0")
+    wd = torch.tensor(w, device=dev)
+    xd = torch.tensor(x, device=dev)
+    rep = s.autotune(len(w), wd.data_ptr(), xd.data_ptr(), reps=3)
+    assert rep.startswith("start")
+    b = s.evaluate(w, x)
+    assert abs(a[0] - b[0]) <= 1e-6 * abs(a[0])
+    assert rel_rms(b[2], a[2]) <= 1e-5 and rel_rms(b[3], a[3]) <= 1e-5
+    ew().tune_clear()
