@@ -538,7 +538,7 @@ def traffic_from_counters(fetch_csv, write_csv, copy_bytes=256 << 20):
     return out
 
 
-def counter_run(systems, steps):
+def counter_run(systems, steps, tune=True):
     import torch
     from ewaldcorrel_amd import ewald
     dev = torch.device("cuda:0")
@@ -548,7 +548,8 @@ def counter_run(systems, steps):
     for f, inp in enumerate(inputs):
         batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
                   inp.dx.data_ptr(), inp.phi.data_ptr())
-    batch.autotune(reps=5)  # the launch shapes the bench measures
+    if tune:
+        batch.autotune(reps=5)  # the launch shapes the bench measures
     for _ in range(steps):
         batch.eval(None)
     batch.sync()
@@ -611,7 +612,7 @@ def main():
     log(f"workload {sy.name}: n={sy.n} K={sy.K} order={sy.order} beta={sy.beta} "
         f"batch={args.batch}")
     if args.counters:
-        counter_run(systems, args.steps)
+        counter_run(systems, args.steps, tune=not args.no_tune)
         return
     s64, b64 = run_precision(systems, "fp64", args.steps, args.warmup, profile=True,
                              tune=not args.no_tune)

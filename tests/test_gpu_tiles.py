@@ -108,8 +108,9 @@ def test_fp64_z_chunks_and_gather_tiles_match_and_meet_the_oracle(case):
 
 
 def test_default_shapes_chunk_long_columns():
-    """Columns too long for whole-column tiles get chunked tiles by default,
-    and the result matches the oracle."""
+    """Columns too long for a whole-column gather tile get chunked gather
+    tiles by default (spreading keeps whole columns), and the result matches
+    the oracle."""
     L6, K, order = np.array([20., 20., 200., 0., 0., 0.]), (16, 16, 384), 6
     rng = np.random.default_rng(9)
     n = 3000
@@ -124,7 +125,7 @@ def test_default_shapes_chunk_long_columns():
         s = ew().Sfac(L6, np.array(K), order, precision=prec)
         shapes = s.launch_shapes()
         say(prec, shapes)
-        assert shapes.split()[4].endswith("x128")  # three spreading chunks
+        assert shapes.split()[4] == "4x4x0"  # whole-column spreading tiles
         if prec == "fp32":
             assert shapes.split()[5] == "8x8x77"  # five gather chunks
         s.set_A(ew().ewald_kernel(0.35), 1.0)
