@@ -321,7 +321,7 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         // planes unrolled for FP32 up to order 4; FP64 and wider stencils keep
         // one plane's n^2 loads in flight (a full unroll needs far more than
         // 128 VGPRs)
-        constexpr int kPlanesUnrolled = sizeof(T) == 4 && ORDER > 0 && ORDER <= 4 ? CAP : 1;
+        constexpr int kPlanesUnrolled = sizeof(T) == 4 && ORDER > 0 && ORDER <= 4 ? 2 : 1;
 #pragma unroll kPlanesUnrolled
         for (int i = 0; i < CAP; ++i) {
             if (i >= n) break;

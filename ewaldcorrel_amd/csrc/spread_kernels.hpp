@@ -127,14 +127,17 @@ __global__ void __launch_bounds__(kBlock) prep_kernel(SPME_FRAME_ARGS) {
 }
 
 // ---- scan: one workgroup per frame ----------------------------------------------------
-// Tiles of 1024 x 16 columns: each thread loads its 16 consecutive counts
-// at once (all loads in flight), the 1024 thread sums are scanned in LDS,
-// and the tile's offsets continue from the running total of earlier tiles.
+// Tiles of 1024 x 16 columns: each thread loads its 16 consecutive counts at
+// once (all loads in flight), the thread sums are scanned within each
+// wavefront by shuffles, the 16 wavefront totals through LDS (two barriers
+// per tile), and the tile's offsets continue from the running total of
+// earlier tiles.
 __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
     constexpr int E = 16;
-    __shared__ int part[1024];
+    __shared__ int wtot[1024 / 64];
     const FrameDesc &F = frames[nf <= 1 ? 0 : blockIdx.x];
     const int ncol = F.K0 * F.K1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int carry = 0;
     for (int base = 0; base < ncol; base += 1024 * E) {
         const int i0 = base + (int)threadIdx.x * E;
@@ -153,16 +156,22 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
         int s = 0;
 #pragma unroll
         for (int e = 0; e < E; ++e) s += v[e];
-        part[threadIdx.x] = s;
-        __syncthreads();
-        // Hillis-Steele inclusive scan over the 1024 thread sums
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int t = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += t;
-            __syncthreads();
+        int incl = s;  // inclusive scan within the wavefront
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
         }
-        int run = carry + (threadIdx.x > 0 ? part[threadIdx.x - 1] : 0);
+        if (lane == 63) wtot[w] = incl;
+        __syncthreads();
+        int woff = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < 1024 / 64; ++q) {
+            const int c = wtot[q];
+            woff += q < w ? c : 0;
+            tot += c;
+        }
+        int run = carry + woff + incl - s;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             if (i0 + e < ncol) F.colstart[i0 + e] = run;
@@ -171,8 +180,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
         int4 *dst = reinterpret_cast<int4 *>(F.colcount + i0);
 #pragma unroll
         for (int e4 = 0; e4 < E / 4; ++e4) dst[e4] = make_int4(0, 0, 0, 0);
-        carry += part[1023];
-        __syncthreads();
+        carry += tot;
+        __syncthreads();  // wtot is rewritten by the next tile
     }
     if (threadIdx.x == 0) F.colstart[ncol] = carry;
     if (threadIdx.x == 0 && F.fixed) {
