@@ -138,7 +138,7 @@ struct FrameDesc {
     int *colstart;   // K0*K1 + 1 exclusive prefix
     int *a_col, *a_rank;
     void *recs;      // AtomRec<T>[n] in column order
-    unsigned long long *maxbits;  // max |w| as bits (fixed point)
+    double *wmax;    // max |w| of each prep workgroup (fixed point), reduced by the scan
     double *fxscale;              // fixed-point scale, written by the scan
     double mmax3;
     int fixed;

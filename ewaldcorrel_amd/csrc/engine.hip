@@ -527,11 +527,9 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     const size_t ncount = ((size_t)nrows + 16383) / 16384 * 16384;
     alloc_((void **)&D.colcount, ncount * sizeof(int));
     alloc_((void **)&D.colstart, ((size_t)nrows + 1) * sizeof(int));
-    alloc_((void **)&D.maxbits, 64);
     alloc_((void **)&D.fxscale, 64);
     alloc_((void **)&result_, 8 * sizeof(double));
     HIPC(hipMemsetAsync(D.colcount, 0, ncount * sizeof(int), stream_));
-    HIPC(hipMemsetAsync(D.maxbits, 0, 64, stream_));
     HIPC(hipMemsetAsync(grid_, 0, grid_reals() * tsz, stream_));
     HIPC(hipMemsetAsync(result_, 0, 8 * sizeof(double), stream_));
     D.out = result_;
@@ -682,6 +680,7 @@ void Engine::ensure_atoms(int n) {
     alloc_((void **)&desc_.a_rank, cap * sizeof(int));
     alloc_((void **)&desc_.recs,
            cap * (fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>)));
+    alloc_((void **)&desc_.wmax, ((cap + kBlock - 1) / kBlock) * sizeof(double));
     atom_cap_ = cap;
     const size_t need = 5 * cap * sizeof(double);
     if (need > pin_cap_) {

@@ -94,9 +94,7 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 // int32 (|q| < 2^30, a quantum 2^-30 of the largest one, far below FP32
 // rounding) and a sum of fewer than 2^33 of them fits in int64.  Clamped so
 // that it stays finite when every charge is zero.
-__device__ __forceinline__ double fixed_scale(int n, unsigned long long maxbits, double mmax3,
-                                              int fx32) {
-    const double wmax = __longlong_as_double((long long)maxbits);
+__device__ __forceinline__ double fixed_scale(int n, double wmax, double mmax3, int fx32) {
     const double bound = fmax(fx32 ? wmax * mmax3 : (double)n * wmax, 1e-200);
     int e;
     frexp(bound, &e);
@@ -119,10 +117,11 @@ __global__ void __launch_bounds__(kBlock) prep_kernel(SPME_FRAME_ARGS) {
         F.a_rank[a] = atomicAdd(F.colcount + col, 1);
         wa = fabs(F.w[a]);
     }
+    // one partial maximum per workgroup (a single atomic address for the
+    // whole frame serialises hundreds of workgroups at memory)
     if (F.fixed) {
         const double m = block_max(wa, red);
-        if (threadIdx.x == 0 && m > 0.0)
-            atomicMax(F.maxbits, (unsigned long long)__double_as_longlong(m));
+        if (threadIdx.x == 0) F.wmax[b] = m;
     }
 }
 
@@ -184,9 +183,18 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
         __syncthreads();  // wtot is rewritten by the next tile
     }
     if (threadIdx.x == 0) F.colstart[ncol] = carry;
-    if (threadIdx.x == 0 && F.fixed) {
-        *F.fxscale = fixed_scale(F.n, *F.maxbits, F.mmax3, F.fx32);
-        *F.maxbits = 0ull;
+    if (F.fixed) {  // max |w| over the prep workgroups' partial maxima
+        __shared__ double wred[1024 / 64];
+        const int nb = (F.n + kBlock - 1) / kBlock;
+        double m = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, F.wmax[i]);
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if (lane == 0) wred[w] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < 1024 / 64; ++q) m = fmax(m, wred[q]);
+            *F.fxscale = fixed_scale(F.n, m, F.mmax3, F.fx32);
+        }
     }
 }
 
