@@ -269,6 +269,7 @@ def run_precision(systems, precision, steps, warmup, profile, tune=True, split=(
             sb.close()
     bat["table_mode"] = handles[0].info()["table_mode"]
     bat["E"] = [float(inp.out[0].item()) for inp in inputs]
+    bat["vir"] = [inp.out[1:7].cpu().numpy().copy() for inp in inputs]
     bat["dx0"] = inputs[0].dx.cpu().numpy()
     bat["phi0"] = inputs[0].phi.cpu().numpy()
     batch.close()
@@ -342,7 +343,8 @@ def large_box(steps, warmup, tune=True):
                      "shapes": h.launch_shapes()}
             log(f"large box {prec} autotune: {lines[-1]}; {tuned['shapes']}")
         r = time_single(sy, h, inp, steps, warmup)
-        outs[prec] = (float(inp.out[0].item()), inp.dx.cpu().numpy().copy())
+        outs[prec] = (float(inp.out[0].item()), inp.dx.cpu().numpy().copy(),
+                      inp.out[1:7].cpu().numpy().copy(), inp.phi.cpu().numpy().copy())
         h.profile(True)
         for _ in range(3):
             h.eval_dev(inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
@@ -357,10 +359,13 @@ def large_box(steps, warmup, tune=True):
                                         for k, v in prof.items()},
                      "autotune": tuned}
         del h
-    (e32, f32), (e64, f64) = outs["fp32"], outs["fp64"]
+    (e32, f32, v32, p32), (e64, f64, v64, p64) = outs["fp32"], outs["fp64"]
     res["fp32_error_vs_fp64_engine"] = {
         "energy_rel": abs(e32 - e64) / abs(e64),
-        "force_rel_rms": float(np.sqrt(np.mean((f32 - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2)))}
+        "virial_rel_max": float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64))),
+        "force_rel_rms": float(np.sqrt(np.mean((f32 - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
+        "potential_rel_rms": float(np.sqrt(np.mean((p32 - p64) ** 2)) /
+                                   np.sqrt(np.mean(p64 ** 2)))}
     return res
 
 
@@ -629,6 +634,8 @@ def main():
                np.sqrt(np.mean(b64["dx0"] ** 2)))
     ePhi = float(np.sqrt(np.mean((b32["phi0"] - b64["phi0"]) ** 2)) /
                  np.sqrt(np.mean(b64["phi0"] ** 2)))
+    eVir = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+               for a, b in zip(b32["vir"], b64["vir"]))
 
     # dominant kernel of the batched FP32 engine, timed live with HIP events
     B = args.batch
@@ -712,8 +719,11 @@ def main():
                  "dtype": "f64",
                  "eval_roofline_frac": Be64 / (b64["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "speedup_fp32_vs_fp64": b32["evals_per_s"] / b64["evals_per_s"],
-        "fp32_error_vs_fp64_engine": {"energy_rel_max": eE, "force_rel_rms": eF,
-                                      "potential_rel_rms": ePhi},
+        "fp32_error_vs_fp64_engine": {"energy_rel_max": eE, "virial_rel_max": eVir,
+                                      "force_rel_rms": eF, "potential_rel_rms": ePhi,
+                                      "over": "every frame's energy and virial (max "
+                                              "component error over the largest), frame "
+                                              "0's forces and potentials"},
         "eval_roofline": {"algorithmic_bytes_per_step": Be32,
                           "achieved_GBs": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9,
                           "frac": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
