@@ -161,21 +161,24 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             launch_x<T, 2>(L, g, bl, lds, st, -1, 0);
             break;
         case ST_YZ_FWD:
+        case ST_YZ_INV: {
+            auto go = [&](auto ept_c) {
+                constexpr int E = decltype(ept_c)::value;
+                if (s == ST_YZ_FWD)
+                    fft_yz_fwd_kernel<T, E><<<g, slab_block<T, E>(), lds, st>>>(L.frames, L.offs,
+                                                                               L.nf);
+                else
+                    fft_yz_inv_kernel<T, E><<<g, slab_block<T, E>(), lds, st>>>(
+                        L.frames, L.offs, L.nf, finish);
+            };
             if (L.slab_ept == SlabCfg<T>::small)
-                fft_yz_fwd_kernel<T, SlabCfg<T>::small>
-                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf);
+                go(std::integral_constant<int, SlabCfg<T>::small>{});
+            else if (L.slab_ept == SlabCfg<T>::medium)
+                go(std::integral_constant<int, SlabCfg<T>::medium>{});
             else
-                fft_yz_fwd_kernel<T, SlabCfg<T>::large>
-                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf);
+                go(std::integral_constant<int, SlabCfg<T>::large>{});
             break;
-        case ST_YZ_INV:
-            if (L.slab_ept == SlabCfg<T>::small)
-                fft_yz_inv_kernel<T, SlabCfg<T>::small>
-                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf, finish);
-            else
-                fft_yz_inv_kernel<T, SlabCfg<T>::large>
-                    <<<g, SlabCfg<T>::block, lds, st>>>(L.frames, L.offs, L.nf, finish);
-            break;
+        }
         case ST_INFL: influence_kernel<T, true><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_INFL_E: influence_kernel<T, false><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
         case ST_XI:
@@ -250,9 +253,11 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        const void *fns[4] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
+        const void *fns[6] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
+                              (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::medium>,
                               (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::large>,
                               (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::small>,
+                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::medium>,
                               (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::large>};
         ok = true;
         for (const void *fn : fns)

@@ -23,16 +23,33 @@
 
 namespace spme {
 
-// FP32 planes (<= 80 KB: two workgroups per CU) use 512 threads held to
-// 128 VGPRs so that both fit; FP64 planes (one workgroup per CU by LDS) use
-// 1024 threads.
+// Variants by elements per thread (EPT), each compiled separately. FP32:
+// planes of up to 768 x 11 elements (128 x 128 needs 8,320) use 768 threads
+// held to 80 VGPRs, so that two workgroups (24 waves) share a CU; up to
+// 512 x 20 elements 512 threads held to 128 VGPRs; larger planes 512 threads
+// with 32 elements each (these spill). FP64 planes (one workgroup per CU by
+// LDS) use 1024 threads.
 template <typename T> struct SlabCfg;
 template <> struct SlabCfg<float> {
-    static constexpr int block = 512, small = 20, large = 32;
+    static constexpr int small = 11, medium = 20, large = 32;
+    static constexpr int block_small = 768, block_medium = 512, block_large = 512;
+    static constexpr int waves_small = 6, waves_medium = 4, waves_large = 4;
 };
-template <> struct SlabCfg<double> {
-    static constexpr int block = 1024, small = 10, large = 20;
+template <> struct SlabCfg<double> {  // medium and large are one variant
+    static constexpr int small = 10, medium = 20, large = 20;
+    static constexpr int block_small = 1024, block_medium = 1024, block_large = 1024;
+    static constexpr int waves_small = 4, waves_medium = 4, waves_large = 4;
 };
+template <typename T, int EPT> constexpr int slab_block() {
+    return EPT == SlabCfg<T>::small    ? SlabCfg<T>::block_small
+           : EPT == SlabCfg<T>::medium ? SlabCfg<T>::block_medium
+                                       : SlabCfg<T>::block_large;
+}
+template <typename T, int EPT> constexpr int slab_waves() {
+    return EPT == SlabCfg<T>::small    ? SlabCfg<T>::waves_small
+           : EPT == SlabCfg<T>::medium ? SlabCfg<T>::waves_medium
+                                       : SlabCfg<T>::waves_large;
+}
 // dynamic LDS budget: gfx950 has 160 KB per workgroup, 1 KB of which is
 // left to the static arrays of the reductions
 constexpr size_t kSlabLds = 159 * 1024;
@@ -44,11 +61,14 @@ __host__ __device__ inline int slab_rows(int K1) { return (K1 + 1) & ~1; }
 // workgroup's threads; the elements per thread decide the variant (small or
 // large, compiled separately; 0: the slab does not apply).
 template <typename T> __host__ __device__ inline int slab_ept_for(int K1, int K2, int ldim) {
-    constexpr int B = SlabCfg<T>::block;
     const int npair = (K1 + 1) / 2;
-    const int need = (max(npair * K2, ldim * K1) + B - 1) / B;
-    return need <= SlabCfg<T>::small ? SlabCfg<T>::small
-                                     : need <= SlabCfg<T>::large ? SlabCfg<T>::large : 0;
+    const int elems = max(npair * K2, ldim * K1);
+    constexpr int Bs = SlabCfg<T>::block_small, Bm = SlabCfg<T>::block_medium,
+                  Bl = SlabCfg<T>::block_large;
+    if ((elems + Bs - 1) / Bs <= SlabCfg<T>::small) return SlabCfg<T>::small;
+    if ((elems + Bm - 1) / Bm <= SlabCfg<T>::medium) return SlabCfg<T>::medium;
+    if ((elems + Bl - 1) / Bl <= SlabCfg<T>::large) return SlabCfg<T>::large;
+    return 0;
 }
 
 template <typename T>
@@ -64,11 +84,11 @@ __device__ __forceinline__ void slab_twiddles(cplx<T> *twz, cplx<T> *twy, const 
 }
 
 template <typename T, int EPT>
-__global__ void __launch_bounds__(SlabCfg<T>::block)
-    __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((slab_block<T, EPT>()))
+    __attribute__((amdgpu_waves_per_eu((slab_waves<T, EPT>()))))
     fft_yz_fwd_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int kSlabBlock = SlabCfg<T>::block;
+    constexpr int kSlabBlock = slab_block<T, EPT>();
     constexpr int U = 8;  // staging registers per round
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
@@ -143,11 +163,11 @@ __global__ void __launch_bounds__(SlabCfg<T>::block)
 // finish 1: workgroup 0 of each frame first finalizes the reduction of the
 // fused x pass that ran before.
 template <typename T, int EPT>
-__global__ void __launch_bounds__(SlabCfg<T>::block)
-    __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((slab_block<T, EPT>()))
+    __attribute__((amdgpu_waves_per_eu((slab_waves<T, EPT>()))))
     fft_yz_inv_kernel(SPME_FRAME_ARGS, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int kSlabBlock = SlabCfg<T>::block;
+    constexpr int kSlabBlock = slab_block<T, EPT>();
     constexpr int U = 8;
     int x;
     const FrameDesc &F = frames[find_frame(offs, nf, x)];
