@@ -119,12 +119,11 @@ void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int 
 template <typename T, int ORDER, int MZ>
 void gather_tile_launch(const Launch &L, dim3 g, size_t lds, hipStream_t st,
                         const SplineArgs<T> &sp) {
+    constexpr int B = gather_block<T, ORDER>();
     if (L.gzchunk)
-        gather_tile_kernel<T, ORDER, MZ, true><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs,
-                                                                              L.nf, sp);
+        gather_tile_kernel<T, ORDER, MZ, true><<<g, B, lds, st>>>(L.frames, L.offs, L.nf, sp);
     else
-        gather_tile_kernel<T, ORDER, MZ, false><<<g, kGatherBlock, lds, st>>>(L.frames, L.offs,
-                                                                               L.nf, sp);
+        gather_tile_kernel<T, ORDER, MZ, false><<<g, B, lds, st>>>(L.frames, L.offs, L.nf, sp);
 }
 
 template <typename T, int ORDER>
@@ -507,7 +506,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     }
     if (fp32 && D.gtx == 0 && K[2] >= 2 * order) {
         const int tx = std::min(8, K[0]), ty = std::min(8, K[1]);
-        const int zlds = (int)((spread_budget - kGatherBlock * sizeof(int)) /
+        const int zlds = (int)((spread_budget - kGatherBlockMax * sizeof(int)) /
                                ((size_t)(tx + order) * (ty + order) * tsz));
         const int zmax = std::min(64 * kGatherMaxZ, zlds) - (order - 1);
         if (tx * ty >= std::min(16, K[0] * K[1]) && zmax >= order) {
@@ -744,7 +743,7 @@ size_t Engine::stage_lds(Stage s) const {
         case ST_GATHER_TILE:  // chunked tiles: the compacted atom list after the rows
             return gather_tile_lds(D.gtx, D.gty, D.gntz > 1 ? D.gtz : K[2], order,
                                    fp32 ? 4 : 8) +
-                   (D.gntz > 1 ? kGatherBlock * sizeof(int) : 0);
+                   (D.gntz > 1 ? kGatherBlockMax * sizeof(int) : 0);
 
         case ST_ZR2C:
         case ST_ZC2R: return lds_fft(fp32, K[2], D.zC, D.plan2.generic, false);
@@ -1041,7 +1040,7 @@ bool gather_tile_fits(int tx, int ty, int tz, const int K[3], int order, size_t 
     const int zl = tz ? tz : K[2];
     if (zl + order - 1 > 64 * kGatherMaxZ) return false;
     if ((tx + order) * (ty + order) > 64 * (kGatherBlock / 64)) return false;  // staged rows
-    return gather_tile_lds(tx, ty, zl, order, tsz) + (tz ? kGatherBlock * sizeof(int) : 0) <=
+    return gather_tile_lds(tx, ty, zl, order, tsz) + (tz ? kGatherBlockMax * sizeof(int) : 0) <=
            lds_max;
 }
 }  // namespace

@@ -179,14 +179,26 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(SPME_FRAME_ARGS, SplineA
 // (P - n/2) mod K2), is staged in LDS with coalesced row loads; then one thread
 // per atom sums its whole stencil from LDS: the n^2 scattered global reads per
 // atom become LDS reads, and each grid row is fetched from L2 once per tile.
+// Workgroup size of the tiled gather: 1024 threads (held to 64 VGPRs, so that
+// two workgroups fill a CU with 32 waves) for FP32 order 4, whose stencil
+// sums fit; 512 threads held to 128 VGPRs otherwise.  Host-side limits use
+// the smaller size (rows staged) and the larger one (the compacted atom list).
 constexpr int kGatherBlock = 512;
+constexpr int kGatherBlockMax = 1024;
+template <typename T, int ORDER> constexpr int gather_block() {
+    return sizeof(T) == 4 && ORDER == 4 ? 1024 : 512;
+}
+template <typename T, int ORDER> constexpr int gather_waves() {
+    return gather_block<T, ORDER>() == 1024 ? 8 : 4;
+}
 constexpr int kGatherLoadRegs = 24;     // staged values each lane holds before storing
 constexpr int kGatherMaxZ = 4;          // row length K2 + n - 1 <= 64 kGatherMaxZ
 
 // MZ = ceil((K2 + n - 1) / 64) row chunks per lane; ZCH: launches with
 // z-chunked frames
 template <typename T, int ORDER, int MZ, bool ZCH>
-__global__ void __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((gather_block<T, ORDER>()))
+    __attribute__((amdgpu_waves_per_eu((gather_waves<T, ORDER>()))))
 gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int b;
@@ -374,8 +386,8 @@ gather_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         // One atom per thread and round; in a z chunk, the round's atoms of the
         // chunk are first compacted into clist, so that the wavefronts that do
         // stencil work are full.
-        __shared__ int wsum[kGatherBlock / 64];
-        int *clist = reinterpret_cast<int *>(tile + rows * Kp);  // kGatherBlock ints
+        __shared__ int wsum[gather_block<T, ORDER>() / 64];
+        int *clist = reinterpret_cast<int *>(tile + rows * Kp);  // kGatherBlockMax ints
         for (int ub = 0; ub < units; ub += blockDim.x) {
             int ridx = -1;
             const int u = ub + (int)threadIdx.x;
