@@ -734,8 +734,11 @@ int Engine::stage_blocks(Stage s) const {
     }
 }
 
-size_t Engine::stage_lds(Stage s) const {
+size_t Engine::stage_lds(Stage s, int xpath) const {
     const FrameDesc &D = desc_;
+    // the x launches of a batch run the most general path of its frames: a
+    // frame with a plain plan in the generic kernel needs its second line set
+    const int xgen = std::max(xpath, x_path(D)) == kXGeneric;
     switch (s) {
         case ST_SPREAD:  // chunked tiles: the compacted atom list after the cells
             return (size_t)D.tx * D.ty * (D.ntz > 1 ? D.tz : K[2] + order - 1) * 8 +
@@ -751,12 +754,11 @@ size_t Engine::stage_lds(Stage s) const {
         case ST_YI: return lds_fft(fp32, K[1], D.yC, D.plan1.generic, true);
         case ST_XF_SCALE:
         case ST_XI:
-            return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, D.plan0.generic)
-                        : xlines_lds_bytes<double>(K[0], D.xC, D.plan0.generic);
+            return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
+                        : xlines_lds_bytes<double>(K[0], D.xC, xgen);
         case ST_XFUSED:  // the block reduction reuses the LDS
-            return std::max(kPartialScratch,
-                            fp32 ? xlines_lds_bytes<float>(K[0], D.xC, D.plan0.generic)
-                                 : xlines_lds_bytes<double>(K[0], D.xC, D.plan0.generic));
+            return std::max(kPartialScratch, fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
+                                                  : xlines_lds_bytes<double>(K[0], D.xC, xgen));
         case ST_YZ_FWD:
         case ST_YZ_INV:
             return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
@@ -1571,7 +1573,7 @@ void Batch::upload_(hipStream_t st) {
         for (int f = 0; f < nf; ++f) {
             offs_[(size_t)s * (nf + 1) + f] = acc;
             acc += frames_[f]->stage_blocks((Stage)s);
-            lds_[s] = std::max(lds_[s], frames_[f]->stage_lds((Stage)s));
+            lds_[s] = std::max(lds_[s], frames_[f]->stage_lds((Stage)s, xpath_));
         }
         offs_[(size_t)s * (nf + 1) + nf] = acc;
         total_[s] = acc;

@@ -182,7 +182,9 @@ public:
     const FrameDesc &desc() const { return desc_; }
     FrameDesc &desc_mut() { return desc_; }
     int stage_blocks(Stage s) const;
-    size_t stage_lds(Stage s) const;
+    // xpath: the x path of the launch (kXBlock, or kXGeneric for a batch with
+    // a generic frame); the frame's own path when that is more general
+    size_t stage_lds(Stage s, int xpath = 0) const;
     void ensure_atoms(int n);
     void launch(Stage s, const Launch &L, int blocks, size_t lds,
                 hipStream_t st, int finish = 0);

@@ -114,26 +114,20 @@ __device__ __forceinline__ void xline_init(XLineF32 &L, const FrameDesc &F, int 
     L.max_m2f = (float)F.max_m2;
 }
 
-// A and dA of the point kx of the line (A, dA as ewald_point_impl, in FP32)
-__device__ __forceinline__ void xline_point(const XLineF32 &L, const FrameDesc &F, int kx,
-                                            float &A, float *dA) {
+// Point kx of the line, accumulated: returns a = sum over the point's
+// images of f = exp(-pi^2 m^2 / beta^2) / (pi m^2) and adds sq fp m_a m_b
+// (fp = (pi^2 / beta^2 - 1 / m^2) f, the derivative terms of
+// ewald_point_impl) to d0..d5 (xx, yy, zz, xy, xz, yz); the plane's
+// multiplicity is left to the caller.  Points on no Nyquist plane have one
+// image and skip the image loop.
+__device__ __forceinline__ float xline_point_acc(const XLineF32 &L, const FrameDesc &F, int kx,
+                                                 float sq, float &d0, float &d1, float &d2,
+                                                 float &d3, float &d4, float &d5) {
     const int nx = kx <= L.half0 ? kx : kx - L.K0;
     const bool xny = L.even0 && kx == L.half0;
-    const int nimg = (xny ? 2 : 1) * (L.yny ? 2 : 1) * (L.zn2 ? 2 : 1);
     const float X0 = (float)nx * L.iL00, X1 = (float)nx * L.iL10, X2 = (float)nx * L.iL20;
-    float a = 0.0f, d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, d4 = 0.0f, d5 = 0.0f;
-    for (int k = 0; k < nimg; ++k) {
-        int kk = k;
-        int sx = 1, sy = 1, sz = 1;
-        if (xny) {
-            sx = (kk & 1) ? -1 : 1;
-            kk >>= 1;
-        }
-        if (L.yny) {
-            sy = (kk & 1) ? -1 : 1;
-            kk >>= 1;
-        }
-        if (L.zn2) sz = (kk & 1) ? -1 : 1;
+    float a = 0.0f;
+    auto image = [&](int sx, int sy, int sz) {
         const float fx = (float)sx, fy = (float)sy, fz = (float)sz;
         const float mx = fx * X0;
         const float my = fx * X1 + fy * L.Y1;
@@ -153,24 +147,37 @@ __device__ __forceinline__ void xline_point(const XLineF32 &L, const FrameDesc &
         if (in) {
             const float r = __builtin_amdgcn_rcpf(m2);
             const float f = __builtin_amdgcn_exp2f(m2 * L.fac2) * r * 0.318309886183790672f;
-            const float fp = (L.fac - r) * f;
-            const float fpx = fp * mx, fpy = fp * my;
+            const float sfp = sq * ((L.fac - r) * f);
+            const float px = sfp * mx, py = sfp * my;
             a += f;
-            d0 += fpx * mx;
-            d1 += fpy * my;
-            d2 += fp * mz * mz;
-            d3 += fpy * mx;
-            d4 += fpx * mz;
-            d5 += fpy * mz;
+            d0 += px * mx;
+            d1 += py * my;
+            d2 += sfp * mz * mz;
+            d3 += py * mx;
+            d4 += px * mz;
+            d5 += py * mz;
         }
+    };
+    if (!(xny || L.yny || L.zn2)) {
+        image(1, 1, 1);
+        return a;
     }
-    A = 0.5f * L.mult * a;
-    dA[0] = L.mult * d0;
-    dA[1] = L.mult * d1;
-    dA[2] = L.mult * d2;
-    dA[3] = L.mult * d3;
-    dA[4] = L.mult * d4;
-    dA[5] = L.mult * d5;
+    const int nimg = (xny ? 2 : 1) * (L.yny ? 2 : 1) * (L.zn2 ? 2 : 1);
+    for (int k = 0; k < nimg; ++k) {
+        int kk = k;
+        int sx = 1, sy = 1, sz = 1;
+        if (xny) {
+            sx = (kk & 1) ? -1 : 1;
+            kk >>= 1;
+        }
+        if (L.yny) {
+            sy = (kk & 1) ? -1 : 1;
+            kk >>= 1;
+        }
+        if (L.zn2) sz = (kk & 1) ? -1 : 1;
+        image(sx, sy, sz);
+    }
+    return a;
 }
 
 __device__ __forceinline__ void table_point(const FrameDesc &F, long h, int kx, int ky, int kz,
@@ -400,8 +407,95 @@ __host__ __device__ inline int x_path(const FrameDesc &F) {
     return F.plan0.generic ? kXGeneric : kXBlock;
 }
 
+// Thread sums of the fused x pass: FP32 in the FP32 engine (seven registers
+// fewer held through the inverse transform), FP64 otherwise.
+template <typename T>
+using XAcc = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+
+// The influence step of the fused x pass on the transformed lines in LDS:
+// S-hat = S iB0 iB1 iB2, the energy and virial sums of A |S-hat|^2 (thread
+// sums in acc[7]) and S-hat A iB (x2 on the kz = 0 and Nyquist planes, which
+// stand for their Hermitian partners) written back for the inverse.  Thread
+// (c, e0) holds column f = b C + c (ky, kz) and elements e0 + q es.
+template <typename T>
+__device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
+                                            const double *ib0, int LS, int c, int e0, int es,
+                                            int f, bool valid, int ky, int kz, double ibyz,
+                                            XAcc<T> acc[7]) {
+    constexpr int EPT = kXElems / kBlock;
+    const int N = F.plan0.N;
+    const bool even = (F.K2 % 2) == 0;
+    const double two = (kz == 0 || (even && kz == F.ldim - 1)) ? 2.0 : 1.0;
+    if (std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
+        // FP32 engine, closed form: FP32 terms, each thread's sums (at most
+        // kXElems / kBlock points) in FP32, the block sums in FP64.  With a
+        // the sum of f = exp(-pi^2 m^2 / beta^2) / (pi m^2) over the point's
+        // images: A = mult a / 2 and dA = mult sum fp m m^T, and since
+        // mult x two = 2 on every plane, the output factor A iB two is a iB;
+        // mult is applied to the thread's sums at the end.
+        const float ibyzf = (float)ibyz;
+        XLineF32 L;
+        xline_init(L, F, ky, kz);
+        float aE = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
+#pragma unroll 1
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * es;
+            if (e < N && valid) {
+                const float ib = (float)ib0[e] * ibyzf;
+                cplx<T> v = res[c * LS + e];
+                const float re = (float)v.x * ib, im = (float)v.y * ib;  // S-hat
+                const float sq = re * re + im * im;
+                const float a = xline_point_acc(L, F, e, sq, d0, d1, d2, d3, d4, d5);
+                aE += a * sq;
+                const float tt = a * ib;
+                v.x = T(re * tt);
+                v.y = T(im * tt);
+                res[c * LS + e] = v;
+            }
+        }
+        acc[0] = XAcc<T>(0.5f * L.mult * aE);
+        acc[1] = XAcc<T>(L.mult * d0);
+        acc[2] = XAcc<T>(L.mult * d1);
+        acc[3] = XAcc<T>(L.mult * d2);
+        acc[4] = XAcc<T>(L.mult * d3);
+        acc[5] = XAcc<T>(L.mult * d4);
+        acc[6] = XAcc<T>(L.mult * d5);
+    } else {  // FP64 terms and thread sums (the FP32 engine rounds its thread sums)
+        double accd[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * es;
+            if (e < N && valid) {
+                const double ib = ib0[e] * ibyz;
+                cplx<T> v = res[c * LS + e];
+                const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+                const double s = re * re + im * im;
+                double A, dA[6];
+                table_point(F, (long)e * F.K1 * F.ldim + f, e, ky, kz, A, dA);
+                accd[0] += A * s;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) accd[1 + k] += s * dA[k];
+                const double tt = A * ib * two;
+                v.x = T(re * tt);
+                v.y = T(im * tt);
+                res[c * LS + e] = v;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) acc[k] = XAcc<T>(accd[k]);
+    }
+}
+
+// waves per SIMD the x launches are held to (registers): the FP32 fused
+// pass at 72 VGPRs runs seven workgroups per CU (measured on the 128^3
+// batch: 93 us per 16 frames, 97.6 at six, 104 at eight with 7 spills)
+template <typename T, int MODE> constexpr int x_waves() {
+    return MODE == 2 && std::is_same<T, float>::value ? 7 : 1;
+}
+
 template <typename T, int MODE, int XP>
-__global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE>()))))
+    fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int EPT = kXElems / kBlock;
     int b;
@@ -452,74 +546,31 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
     const FastDiv dC = {(unsigned)C, 1u, (unsigned)lg, 1u << 15, 15u + (unsigned)lg};
     cplx<T> *res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
     if (MODE == 2) {
-        const int cl = c, ie0 = e0, ies = es;
-        const int f = b * C + cl;
+        const int f = b * C + c;
         const bool valid = f < P;
         const int fv = valid ? f : P - 1;
         const int ky = fdiv(fv, F.dldim), kz = fv - ky * ldim;
-        const double ibyz = F.iB1[ky] * F.iB2[kz];
-        const bool even = (F.K2 % 2) == 0;
-        const double two = (kz == 0 || (even && kz == ldim - 1)) ? 2.0 : 1.0;
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
-            // FP32 engine, closed form: FP32 terms, each thread's sums (at most
-            // kXElems / kBlock points) in FP32, the block sums in FP64
-            const float ibyzf = (float)ibyz, twof = (float)two;
-            XLineF32 L;
-            xline_init(L, F, ky, kz);
-            float accf[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 1
-            for (int q = 0; q < EPT; ++q) {
-                const int e = ie0 + q * ies;
-                if (e < N && valid) {
-                    const float ib = (float)ib0[e] * ibyzf;
-                    cplx<T> v = res[cl * LS + e];
-                    const float re = (float)v.x * ib, im = (float)v.y * ib;  // S-hat
-                    const float s = re * re + im * im;
-                    float A, dA[6];
-                    xline_point(L, F, e, A, dA);
-                    accf[0] += A * s;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) accf[1 + k] += s * dA[k];
-                    const float tt = A * ib * twof;
-                    v.x = T(re * tt);
-                    v.y = T(im * tt);
-                    res[cl * LS + e] = v;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 7; ++k) acc[k] = (double)accf[k];
-        } else {
-#pragma unroll 1
-            for (int q = 0; q < EPT; ++q) {
-                const int e = ie0 + q * ies;
-                if (e < N && valid) {
-                    const double ib = ib0[e] * ibyz;
-                    cplx<T> v = res[cl * LS + e];
-                    const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
-                    const double s = re * re + im * im;
-                    double A, dA[6];
-                    table_point(F, (long)e * P + f, e, ky, kz, A, dA);
-                    acc[0] += A * s;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) acc[1 + k] += s * dA[k];
-                    const double tt = A * ib * two;
-                    v.x = T(re * tt);
-                    v.y = T(im * tt);
-                    res[cl * LS + e] = v;
-                }
-            }
-        }
+        XAcc<T> acc[7];
+        x_influence<T>(F, res, ib0, LS, c, e0, es, f, valid, ky, kz,
+                       F.iB1[ky] * F.iB2[kz], acc);
         __syncthreads();
         res = x_transform<T, XP == kXGeneric>(res, res == lines ? lines2 : lines, C, dC, LS, p,
                                               twl, +1);
-        const bool vs = b * C + c < P;
+        // the store addresses are recomputed here rather than kept (in
+        // registers, or spilled) from the loads at the start
+        int cs = c, es0 = e0;
+        asm volatile("" : "+v"(cs), "+v"(es0));
+        const bool vs = b * C + cs < P;
+        cplx<T> *cols = (cplx<T> *)F.grid + min(b * C + cs, P - 1);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
-            const int e = e0 + q * es;
-            if (e < N && vs) col[(long)e * P] = res[c * LS + e];
+            const int e = es0 + q * es;
+            if (e < N && vs) cols[(long)e * P] = res[cs * LS + e];
         }
-        block_partial(F, b, acc, reinterpret_cast<double *>(smem_raw));  // LDS >= kPartialScratch
+        double accd[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) accd[k] = (double)acc[k];
+        block_partial(F, b, accd, reinterpret_cast<double *>(smem_raw));  // LDS >= kPartialScratch
         return;
     }
     __syncthreads();
@@ -540,5 +591,7 @@ __global__ void __launch_bounds__(kBlock) fft_x_kernel(SPME_FRAME_ARGS, int sign
         }
     }
 }
+
+
 
 }  // namespace spme

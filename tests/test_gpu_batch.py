@@ -128,15 +128,14 @@ def _batch_systems():
     return frames, tri
 
 
-@pytest.mark.parametrize("precision,fixed", [("fp32", True), ("fp64", True), ("fp64", False)])
-def test_batch_equals_frames_alone(precision, fixed):
-    frames, tri = _batch_systems()
+def _check_batch(systems, precision, fixed, oracle_frames, exact=None):
+    """exact (default: fixed): the batch must reproduce each frame alone
+    bitwise; otherwise within 1e-12 (FP64) or 1e-6 (FP32) relative."""
+    exact = fixed if exact is None else exact
     dev = torch.device("cuda:0")
-    systems = [dict(L6=f.L6, K=f.K, order=4, beta=f.beta, w=f.w, x=f.x) for f in frames]
-    systems.append(tri)
     handles, bufs = [], []
     for sy in systems:
-        h = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision=precision, fixed=fixed)
+        h = ew().Sfac(sy["L6"], np.array(sy["K"]), sy["order"], precision=precision, fixed=fixed)
         h.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
         handles.append(h)
         n = len(sy["w"])
@@ -157,25 +156,50 @@ def test_batch_equals_frames_alone(precision, fixed):
         torch.cuda.synchronize()
         for f, (a, b) in enumerate(zip(alone, bufs)):
             o = b["out"].cpu().numpy()
-            if fixed:
+            if exact:
                 assert o[0] == a[0] and np.array_equal(o[1:7], a[1]), (rep, f)
                 assert np.array_equal(b["dx"].cpu().numpy(), a[2]), (rep, f)
                 assert np.array_equal(b["phi"].cpu().numpy(), a[3]), (rep, f)
             else:
-                assert abs(o[0] - a[0]) <= 1e-13 * abs(a[0]), (rep, f)
-                assert rel(o[1:7], a[1]) <= 1e-12
-                assert rel_rms(b["dx"].cpu().numpy(), a[2]) <= 1e-12
-    # and the batch agrees with the oracle on two frames
-    for f in (0, len(systems) - 1):
+                tol = 1e-12 if precision == "fp64" else 1e-6
+                assert abs(o[0] - a[0]) <= tol * abs(a[0]), (rep, f)
+                assert rel(o[1:7], a[1]) <= 10 * tol
+                assert rel_rms(b["dx"].cpu().numpy(), a[2]) <= 10 * tol
+                assert rel_rms(b["phi"].cpu().numpy(), a[3]) <= 10 * tol
+    # and the batch agrees with the oracle
+    for f in oracle_frames:
         sy = systems[f]
-        o = OracleSfac(sy["L6"], sy["K"], 4)
+        o = OracleSfac(sy["L6"], sy["K"], sy["order"])
         o.set_A(ewald_rad(sy["beta"]), 1.0)
         E0, v0, dx0, phi0 = o.evaluate(sy["w"], sy["x"])
         E = bufs[f]["out"][0].item()
+        dx = bufs[f]["dx"].cpu().numpy()
         tol = 1e-10 if precision == "fp64" else 1e-6
-        say(precision, f, abs(E - E0) / abs(E0))
+        say(precision, f, sy["K"], abs(E - E0) / abs(E0), rel_rms(dx, dx0))
         assert abs(E - E0) <= tol * abs(E0)
+        assert rel_rms(dx, dx0) <= (1e-9 if precision == "fp64" else 1e-5)
     batch.close()
+
+
+@pytest.mark.parametrize("precision,fixed", [("fp32", True), ("fp64", True), ("fp64", False)])
+def test_batch_equals_frames_alone(precision, fixed):
+    frames, tri = _batch_systems()
+    systems = [dict(L6=f.L6, K=f.K, order=4, beta=f.beta, w=f.w, x=f.x) for f in frames]
+    systems.append(dict(tri, order=4))
+    _check_batch(systems, precision, fixed, (0, len(systems) - 1))
+
+
+@pytest.mark.parametrize("precision,fixed", [("fp32", True), ("fp64", True)])
+def test_batch_mixing_plain_and_generic_radix_frames(precision, fixed):
+    """K0 = 14 = 2 x 7 needs the generic x transform, so the whole batch's x
+    launches run it, with its second set of lines in LDS; the frame with the
+    plain K0 = 32 (64 lines of 33) needs twice the LDS it uses alone.  The
+    generic transform rounds differently from the radix-2..8 one the plain
+    frames run alone, so they agree within rounding, not bitwise."""
+    plain = dict(make_sys([20., 16., 14., 0., 0., 0.], (32, 24, 20), 4, 1500, 11), order=4)
+    gen = dict(make_sys([12., 15., 18., 1., -0.5, 0.5], (14, 18, 21), 4, 900, 12), order=4)
+    plain2 = dict(make_sys([18., 18., 18., 0., 0., 0.], (32, 32, 32), 4, 2000, 13), order=4)
+    _check_batch([plain, gen, plain2], precision, fixed, (0, 1, 2), exact=False)
 
 
 def test_batch_rejects_mixed_precision():
