@@ -202,6 +202,21 @@ def test_batch_mixing_plain_and_generic_radix_frames(precision, fixed):
     _check_batch([plain, gen, plain2], precision, fixed, (0, 1, 2), exact=False)
 
 
+@pytest.mark.parametrize("precision,fixed", [("fp32", True)])
+def test_batch_of_many_planes_equals_frames_alone(precision, fixed):
+    """A batch of 672 x planes over frames of two grid shapes (more
+    workgroups per launch than the device holds at once): bitwise the
+    frames alone and within the FP32 bars of the oracle."""
+    systems = []
+    for f in range(8):
+        systems.append(dict(make_sys([22., 18., 16., 0.5, -0.5, 0.3], (64, 48, 40), 4, 1200,
+                                     20 + f), order=4))
+    for f in range(2):
+        systems.append(dict(make_sys([26., 21., 19., 0., 1., 0.], (80, 64, 50), 4, 1800,
+                                     40 + f), order=4))
+    _check_batch(systems, precision, fixed, (0, 8, 9))
+
+
 def test_batch_rejects_mixed_precision():
     a = ew().Sfac(np.array([10., 10, 10, 0, 0, 0]), np.array([16, 16, 16]), 4, precision="fp32")
     b = ew().Sfac(np.array([10., 10, 10, 0, 0, 0]), np.array([16, 16, 16]), 4)
