@@ -268,7 +268,7 @@ __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, i
 // arithmetic uses FastDivs precomputed per stage.  In place, register staged:
 // every thread reads its butterflies' inputs, barrier, writes the outputs.
 // Requires C N <= EPT NT.
-template <typename T, int R, int EPT, int NT>
+template <typename T, int R, int EPT, int NT, bool OPQ>
 __device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, int LST, int EST,
                                        int N, int Ns, const FastDiv &dNs, int twstep,
                                        const cplx<T> *tw, int sign) {
@@ -277,9 +277,15 @@ __device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, i
     const int nb = C * NR;
     cplx<T> v[MAXQ][R];
     int dst[MAXQ];
+    // OPQ: the thread's butterfly indices are recomputed in every stage (an
+    // opaque thread id) instead of being kept from stage to stage, which
+    // costs registers (spilled, or a workgroup less per CU) where registers
+    // bound the kernel and a little issue time where they do not
+    int tid = (int)threadIdx.x;
+    if (OPQ) asm volatile("" : "+v"(tid));
 #pragma unroll
     for (int q = 0; q < MAXQ; ++q) {
-        const int bf = threadIdx.x + q * NT;
+        const int bf = tid + q * NT;
         dst[q] = -1;
         if (bf < nb) {
             const int j = fdiv(bf, dC);
@@ -314,7 +320,7 @@ __device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, i
 }
 
 // All stages of a plan whose radices are 2, 3, 4, 5 and 8 (plan.generic == 0).
-template <typename T, int EPT, int NT>
+template <typename T, int EPT, int NT, bool OPQ = false>
 __device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC, int LST,
                                           int EST, const FftPlanArgs &p, const cplx<T> *tw,
                                           int sign) {
@@ -324,11 +330,11 @@ __device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC
         const FastDiv &dNs = p.dNs[s];
         const int ts = p.twstep[s];
         switch (R) {
-            case 2: lstage<T, 2, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
-            case 3: lstage<T, 3, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
-            case 4: lstage<T, 4, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
-            case 5: lstage<T, 5, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
-            default: lstage<T, 8, EPT, NT>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 2: lstage<T, 2, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 3: lstage<T, 3, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 4: lstage<T, 4, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 5: lstage<T, 5, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            default: lstage<T, 8, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
         }
         Ns *= R;
     }

@@ -390,12 +390,12 @@ __host__ __device__ inline size_t xlines_lds_bytes(int N, int C, int generic) {
 // x transform of the C lines in LDS by the whole workgroup (plans with a
 // generic radix, or fewer than four lines); returns the buffer holding the
 // result
-template <typename T, bool GENERIC>
+template <typename T, bool GENERIC, bool OPQ = false>
 __device__ __forceinline__ cplx<T> *x_transform(cplx<T> *buf, cplx<T> *other, int C,
                                                 const FastDiv &dC, int LS, const FftPlanArgs &p,
                                                 const cplx<T> *tw, int sign) {
     if (GENERIC) return fft_lines<T, kXElems / kBlock>(buf, other, C, LS, p, tw, sign);
-    lines_fft<T, kXElems / kBlock, kBlock>(buf, C, dC, LS, 1, p, tw, sign);
+    lines_fft<T, kXElems / kBlock, kBlock, OPQ>(buf, C, dC, LS, 1, p, tw, sign);
     return buf;
 }
 
@@ -486,12 +486,13 @@ __device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
     }
 }
 
-// waves per SIMD the x launches are held to (registers): the FP32 fused
-// pass at 72 VGPRs runs seven workgroups per CU (measured on the 128^3
-// batch: 93 us per 16 frames, 97.6 at six, 104 at eight with 7 spills)
-template <typename T, int MODE> constexpr int x_waves() {
-    return MODE == 2 && std::is_same<T, float>::value ? 7 : 1;
+// waves per SIMD the fused x launches are held to (registers): FP32 at 64
+// VGPRs (line transforms with per-stage indices) runs eight workgroups per
+// CU (FP64 at 102 would spill)
+template <typename T, int MODE> constexpr bool x_lean() {
+    return MODE == 2 && std::is_same<T, float>::value;
 }
+template <typename T, int MODE> constexpr int x_waves() { return x_lean<T, MODE>() ? 8 : 1; }
 
 template <typename T, int MODE, int XP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE>()))))
@@ -544,7 +545,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
     const int s0 = MODE == 0 ? sign : -1;
     // power of two: n div C = n >> lg (fdiv: mulhi(n, 1) = 0; fdiv15: n 2^15 >> (15 + lg))
     const FastDiv dC = {(unsigned)C, 1u, (unsigned)lg, 1u << 15, 15u + (unsigned)lg};
-    cplx<T> *res = x_transform<T, XP == kXGeneric>(lines, lines2, C, dC, LS, p, twl, s0);
+    cplx<T> *res = x_transform<T, XP == kXGeneric, x_lean<T, MODE>()>(lines, lines2, C, dC, LS, p, twl, s0);
     if (MODE == 2) {
         const int f = b * C + c;
         const bool valid = f < P;
@@ -554,7 +555,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
         x_influence<T>(F, res, ib0, LS, c, e0, es, f, valid, ky, kz,
                        F.iB1[ky] * F.iB2[kz], acc);
         __syncthreads();
-        res = x_transform<T, XP == kXGeneric>(res, res == lines ? lines2 : lines, C, dC, LS, p,
+        res = x_transform<T, XP == kXGeneric, x_lean<T, MODE>()>(res, res == lines ? lines2 : lines, C, dC, LS, p,
                                               twl, +1);
         // the store addresses are recomputed here rather than kept (in
         // registers, or spilled) from the loads at the start

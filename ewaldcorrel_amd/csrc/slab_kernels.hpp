@@ -27,8 +27,8 @@ namespace spme {
 // planes of up to 768 x 11 elements (128 x 128 needs 8,320) use 768 threads
 // held to 80 VGPRs, so that two workgroups (24 waves) share a CU; up to
 // 512 x 20 elements 512 threads held to 128 VGPRs; larger planes 512 threads
-// with 32 elements each (these spill). FP64 planes (one workgroup per CU by
-// LDS) use 1024 threads.
+// with 32 elements each. FP64 planes (one workgroup per CU by LDS) use 1024
+// threads.
 template <typename T> struct SlabCfg;
 template <> struct SlabCfg<float> {
     static constexpr int small = 11, medium = 20, large = 32;
@@ -123,7 +123,11 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
         }
     }
     __syncthreads();
-    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, -1);
+    // per-stage butterfly indices where registers bound the kernel (kept
+    // across stages they are spilled); the small FP64 variant has registers
+    // to spare (one workgroup per CU by LDS)
+    constexpr bool lean = std::is_same<T, float>::value || EPT != SlabCfg<T>::small;
+    lines_fft<T, EPT, kSlabBlock, lean>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, -1);
     // untangle the pairs into rows 2c and 2c+1 (all reads before the barrier)
     {
         constexpr int Q = EPT / 2 + 1;
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
         __syncthreads();
     }
     // y transform: ldim lines (columns kz), elements at stride RS
-    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, -1);
+    lines_fft<T, EPT, kSlabBlock, lean>(slab, RS, F.dldim, 1, RS, F.plan1, twy, -1);
     cplx<T> *out = (cplx<T> *)F.grid + (long)x * K1 * RS;
     for (int l = threadIdx.x; l < K1 * RS; l += kSlabBlock) out[l] = slab[l];
 }
@@ -210,7 +214,10 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
         }
     }
     __syncthreads();
-    lines_fft<T, EPT, kSlabBlock>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
+    // per-stage butterfly indices in the larger variants (kept across stages
+    // they are spilled); the small ones run faster keeping them
+    constexpr bool lean = EPT != SlabCfg<T>::small;
+    lines_fft<T, EPT, kSlabBlock, lean>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
     // Z_k = A_k + i B_k and the Hermitian completion (all reads before the barrier)
     {
         constexpr int Q = EPT / 2 + 1;
@@ -250,7 +257,7 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
         }
         __syncthreads();
     }
-    lines_fft<T, EPT, kSlabBlock>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, +1);
+    lines_fft<T, EPT, kSlabBlock, lean>(slab, npair, F.dpair, 2 * RS, 1, F.plan2, twz, +1);
     T *g = (T *)F.grid + (long)x * K1 * 2 * RS;
     const int nz = npair * K2;
     for (int l = threadIdx.x; l < nz; l += kSlabBlock) {
