@@ -86,6 +86,28 @@ def test_slab_equals_line_passes(grid, precision, fixed):
             assert np.array_equal(np.asarray(u), np.asarray(v))
 
 
+def test_fp32_largest_slab_variant():
+    """FP32 planes of more than 512 x 20 elements run the 512-thread variant
+    with 32 elements per thread (128 x 97 = 12,416 here; FP64 planes this
+    large do not fit in LDS): slab and line passes agree and meet the
+    oracle."""
+    sy = make_sys([10., 40., 60., 0.5, -1., 2.], (8, 128, 192), 4, 2500, 7)
+    s = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision="fp32")
+    s.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+    assert s.info()["slab"] == 1
+    a = evaluate(s, sy)
+    s.set_transform(1)
+    b = evaluate(s, sy)
+    o = OracleSfac(sy["L6"], sy["K"], 4)
+    o.set_A(ewald_rad(sy["beta"]), 1.0)
+    E0, v0, dx0, phi0 = o.evaluate(sy["w"], sy["x"])
+    say("largest slab fp32: E", abs(a[0] - b[0]) / abs(b[0]), "oracle", abs(a[0] - E0) / abs(E0),
+        "F", rel_rms(a[2], dx0))
+    assert abs(a[0] - b[0]) <= 1e-6 * abs(b[0]) and rel_rms(a[2], b[2]) <= 1e-5
+    assert abs(a[0] - E0) <= 1e-6 * abs(E0) and rel(a[1], v0) <= 1e-5
+    assert rel_rms(a[2], dx0) <= 1e-5 and rel_rms(a[3], phi0) <= 1e-5
+
+
 @pytest.mark.parametrize("mode", [0, 1], ids=["slab", "lines"])
 def test_split_api_and_get_S_against_oracle(mode):
     name, L6, K, order, N = GRIDS[0]
