@@ -83,6 +83,30 @@ __device__ __forceinline__ void slab_twiddles(cplx<T> *twz, cplx<T> *twy, const 
     for (int t = threadIdx.x; t < F.K1; t += (int)blockDim.x) twy[t] = gy[t];
 }
 
+// bytes (a multiple of 16) from src (16-B aligned) to LDS dst (16-B aligned)
+// by direct global -> LDS loads, 16 B per lane: wave w copies the 1 KB chunks
+// w, w + nwaves, ...  Issued from asm so that the compiler does not wait for
+// them before unrelated LDS reads; the caller waits (vmcnt 0) and then
+// synchronizes the workgroup before reading dst.
+__device__ __forceinline__ void lds_stream16(const void *src, void *dst, int bytes) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int nw = (int)blockDim.x >> 6;
+    const char *sp = (const char *)src;
+    const unsigned base =
+        (unsigned)(size_t)(__attribute__((address_space(3))) char *)((char *)dst);
+    for (int off = wv * 1024; off < bytes; off += nw * 1024) {
+        const unsigned m0 = __builtin_amdgcn_readfirstlane(base + (unsigned)off);
+        if (off + lane * 16 < bytes) {
+            const char *gp = sp + off + lane * 16;
+            asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                         :
+                         : "s"(m0), "v"(gp)
+                         : "memory", "m0");
+        }
+    }
+}
+
 template <typename T, int EPT>
 __global__ void __launch_bounds__((slab_block<T, EPT>()))
     __attribute__((amdgpu_waves_per_eu((slab_waves<T, EPT>()))))
@@ -187,22 +211,32 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
     slab_twiddles<T>(twz, twy, F);
     const cplx<T> *in = (const cplx<T> *)F.grid + (long)x * K1 * RS;
     const int tot = K1 * RS;
-    for (int base = 0; base < tot; base += U * kSlabBlock) {
-        T vx[U], vy[U];
+    const int bytes = tot * (int)sizeof(cplx<T>);
+    if (((size_t)slab & 15) == 0 && ((size_t)in & 15) == 0 && (bytes & 15) == 0) {
+        // the whole plane in flight at once (direct global -> LDS loads, no
+        // staging registers): 2-3 us less per 16-frame launch than staging
+        // through registers in rounds; the forward kernel's paired rows, at
+        // 4 B per lane, measured slower and keep the staged loads
+        lds_stream16(in, slab, bytes);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        for (int base = 0; base < tot; base += U * kSlabBlock) {
+            T vx[U], vy[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const cplx<T> v = in[min(base + (int)threadIdx.x + q * kSlabBlock, tot - 1)];
-            vx[q] = v.x;
-            vy[q] = v.y;
-        }
+            for (int q = 0; q < U; ++q) {
+                const cplx<T> v = in[min(base + (int)threadIdx.x + q * kSlabBlock, tot - 1)];
+                vx[q] = v.x;
+                vy[q] = v.y;
+            }
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int l = base + (int)threadIdx.x + q * kSlabBlock;
-            if (l < tot) {
-                cplx<T> v;
-                v.x = vx[q];
-                v.y = vy[q];
-                slab[l] = v;
+            for (int q = 0; q < U; ++q) {
+                const int l = base + (int)threadIdx.x + q * kSlabBlock;
+                if (l < tot) {
+                    cplx<T> v;
+                    v.x = vx[q];
+                    v.y = vy[q];
+                    slab[l] = v;
+                }
             }
         }
     }
