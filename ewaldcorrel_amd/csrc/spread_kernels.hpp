@@ -137,6 +137,13 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
     const FrameDesc &F = frames[nf <= 1 ? 0 : blockIdx.x];
     const int ncol = F.K0 * F.K1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the prep workgroups' partial maxima of |w|, loaded before the scan so
+    // that their latency overlaps it
+    double m = 0.0;
+    if (F.fixed) {
+        const int nb = (F.n + kBlock - 1) / kBlock;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, F.wmax[i]);
+    }
     int carry = 0;
     for (int base = 0; base < ncol; base += 1024 * E) {
         const int i0 = base + (int)threadIdx.x * E;
@@ -171,10 +178,23 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
             tot += c;
         }
         int run = carry + woff + incl - s;
+        if (i0 + E <= ncol) {  // four 16-byte stores (i0 is a multiple of 16)
+            int o[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (i0 + e < ncol) F.colstart[i0 + e] = run;
-            run += v[e];
+            for (int e = 0; e < E; ++e) {
+                o[e] = run;
+                run += v[e];
+            }
+            int4 *cs = reinterpret_cast<int4 *>(F.colstart + i0);
+#pragma unroll
+            for (int e4 = 0; e4 < E / 4; ++e4)
+                cs[e4] = make_int4(o[4 * e4], o[4 * e4 + 1], o[4 * e4 + 2], o[4 * e4 + 3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (i0 + e < ncol) F.colstart[i0 + e] = run;
+                run += v[e];
+            }
         }
         int4 *dst = reinterpret_cast<int4 *>(F.colcount + i0);
 #pragma unroll
@@ -185,9 +205,6 @@ __global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
     if (threadIdx.x == 0) F.colstart[ncol] = carry;
     if (F.fixed) {  // max |w| over the prep workgroups' partial maxima
         __shared__ double wred[1024 / 64];
-        const int nb = (F.n + kBlock - 1) / kBlock;
-        double m = 0.0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, F.wmax[i]);
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
         if (lane == 0) wred[w] = m;
         __syncthreads();
