@@ -126,6 +126,7 @@ struct FrameDesc {
     // influence function
     const double *table;  // A[H] then dA[6H] (kTableFull)
     int tabmode;
+    int infl32;        // FP32 engine: closed-form influence and thread sums in FP32 (option)
     double ewald_fac;  // -(pi/eta)^2
     double max_m2;
     CellArgs tcell;    // box the influence function was built for (set_L keeps it)

@@ -110,10 +110,23 @@ template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, in
 template <typename T, int MODE>
 void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int sign,
               int finish) {
+    // only the fused pass (MODE 2) applies the influence; the FP32 influence
+    // option of the FP32 engine has its own instantiation
+    if constexpr (MODE == 2 && std::is_same<T, float>::value) {
+        if (L.infl32) {
+        if (L.xpath == kXBlock)
+            fft_x_kernel<T, MODE, kXBlock, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+        else
+            fft_x_kernel<T, MODE, kXGeneric, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
+                                                                   finish);
+        return;
+        }
+    }
     if (L.xpath == kXBlock)
-        fft_x_kernel<T, MODE, kXBlock><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+        fft_x_kernel<T, MODE, kXBlock, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
     else
-        fft_x_kernel<T, MODE, kXGeneric><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
+        fft_x_kernel<T, MODE, kXGeneric, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
+                                                               finish);
 }
 
 template <typename T, int ORDER, int MZ>
@@ -420,6 +433,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     D.mmax3 = mmax3;
     D.fixed = fixed ? 1 : 0;
     D.fx32 = fp32 ? 1 : 0;
+    D.infl32 = (fp32 && (flags & kFlagInfl32)) ? 1 : 0;
     D.tabmode = kTableFull;
     const int nrows = K[0] * K[1];
     D.zC = fft_lines(fp32, K[2], (nrows + 1) / 2, D.plan2.generic, false);
@@ -557,7 +571,7 @@ Engine::~Engine() {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
-    for (void *p : allocs_) (void)hipFree(p);
+    for (auto &a : allocs_) (void)hipFree(a.first);
     if (pin_) (void)hipHostFree(pin_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -571,8 +585,31 @@ void Engine::alloc_(void **p, size_t bytes) {
         throw std::runtime_error("hipMalloc(" + std::to_string(bytes) +
                                  " B) failed: " + hipGetErrorString(e));
     }
-    allocs_.push_back(*p);
+    allocs_.push_back({*p, bytes});
     dev_bytes_ += bytes;
+}
+
+void Engine::free_(void *p) {
+    if (!p) return;
+    for (size_t i = 0; i < allocs_.size(); ++i)
+        if (allocs_[i].first == p) {
+            HIPC(hipFree(p));
+            dev_bytes_ -= allocs_[i].second;
+            allocs_.erase(allocs_.begin() + (long)i);
+            return;
+        }
+}
+
+void Engine::check_staging(int n, std::initializer_list<const void *> ptrs) const {
+    if (n < 0 || (size_t)n <= atom_cap_) return;
+    const void *own[4] = {x_d_, w_d_, dx_d_, phi_d_};
+    for (const void *p : ptrs)
+        for (const void *o : own)
+            if (p && p == o)
+                throw std::invalid_argument(
+                    "an atom buffer of this handle's own staging holds " +
+                    std::to_string(atom_cap_) + " atoms, fewer than n = " + std::to_string(n) +
+                    " (grow it first and take its address afterwards)");
 }
 
 void Engine::drop_graph_() {
@@ -594,6 +631,7 @@ void Engine::set_cell(const double L6_[6]) {
     cell.iL22 = c.iL[2][2];
     cell.iV = c.iV;
     desc_.cell = cell;
+    ++gen_;
     drop_graph_();
 }
 
@@ -607,6 +645,7 @@ void Engine::upload_table(const double *A, const double *dA) {
     desc_.tabmode = kTableFull;
     desc_.tcell = cell;
     has_table_ = true;
+    ++gen_;
     drop_graph_();
 }
 
@@ -655,6 +694,7 @@ void Engine::build_ewald_table(double eta, double max_m) {
     desc_.max_m2 = max_m * max_m;
     desc_.tcell = cell;
     has_table_ = true;
+    ++gen_;
     drop_graph_();
 }
 
@@ -673,18 +713,18 @@ void Engine::ensure_atoms(int n) {
     sync();
     drop_graph_();
     const size_t cap = std::max<size_t>(1024, (size_t)n + n / 4);
-    const size_t tsz = fp32 ? sizeof(float) : sizeof(double);
-    // staging for the host-pointer entry points and the sort workspace; the
-    // old buffers stay allocated until the engine is destroyed (rare growth)
+    // staging for the host-pointer entry points and the sort workspace: new
+    // buffers first, then the old ones are released (the stream is idle);
+    // batches holding the old workspace addresses see the generation change
+    void *old[5] = {x_d_, w_d_, dx_d_, phi_d_, ws_};
+    ++gen_;
     alloc_((void **)&x_d_, 3 * cap * sizeof(double));
     alloc_((void **)&w_d_, cap * sizeof(double));
     alloc_((void **)&dx_d_, 3 * cap * sizeof(double));
     alloc_((void **)&phi_d_, cap * sizeof(double));
-    alloc_((void **)&desc_.a_col, cap * sizeof(int));
-    alloc_((void **)&desc_.a_rank, cap * sizeof(int));
-    alloc_((void **)&desc_.recs,
-           cap * (fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>)));
-    alloc_((void **)&desc_.wmax, ((cap + kBlock - 1) / kBlock) * sizeof(double));
+    alloc_(&ws_, workspace_bytes(cap));
+    bind_workspace(desc_, ws_, cap);
+    for (void *p : old) free_(p);
     atom_cap_ = cap;
     const size_t need = 5 * cap * sizeof(double);
     if (need > pin_cap_) {
@@ -692,6 +732,31 @@ void Engine::ensure_atoms(int n) {
         HIPC(hipHostMalloc((void **)&pin_, need, hipHostMallocDefault));
         pin_cap_ = need;
     }
+}
+
+// The per-atom sort workspace of n atoms in one block: column ids and ranks,
+// the column-ordered records and one |w| maximum per prep workgroup, each
+// part 256-byte aligned.  Nothing in it needs to start at zero.
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t Engine::workspace_bytes(size_t n) const {
+    const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
+    return 2 * align256(n * sizeof(int)) + align256(n * rec) +
+           align256((n + kBlock - 1) / kBlock * sizeof(double));
+}
+
+void Engine::bind_workspace(FrameDesc &D, void *base, size_t n) const {
+    const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
+    char *p = static_cast<char *>(base);
+    D.a_col = reinterpret_cast<int *>(p);
+    p += align256(n * sizeof(int));
+    D.a_rank = reinterpret_cast<int *>(p);
+    p += align256(n * sizeof(int));
+    D.recs = p;
+    p += align256(n * rec);
+    D.wmax = reinterpret_cast<double *>(p);
 }
 
 void Engine::set_atoms_(int n, const double *w, const double *x, double *dx, double *phi,
@@ -704,12 +769,14 @@ void Engine::set_atoms_(int n, const double *w, const double *x, double *dx, dou
     desc_.out = out;
 }
 
-int Engine::stage_blocks(Stage s) const {
+int Engine::stage_blocks(Stage s) const { return stage_blocks_n(s, desc_.n); }
+
+int Engine::stage_blocks_n(Stage s, int natoms) const {
     const FrameDesc &D = desc_;
     const int nrows = K[0] * K[1];
     switch (s) {
         case ST_PREP:
-        case ST_SCATTER: return (D.n + kBlock - 1) / kBlock;
+        case ST_SCATTER: return (natoms + kBlock - 1) / kBlock;
         case ST_SCAN: return 1;
         case ST_SPREAD: return D.ntx * D.nty * D.ntz;
         case ST_ZR2C:
@@ -727,7 +794,7 @@ int Engine::stage_blocks(Stage s) const {
         case ST_GATHER_SORTED:
         case ST_GATHER: {
             const int apb = kBlock / order;
-            return (D.n + apb - 1) / apb;
+            return (natoms + apb - 1) / apb;
         }
         case ST_GATHER_TILE: return D.gntx * D.gnty * D.gntz;
         default: return 0;
@@ -756,9 +823,10 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
         case ST_XI:
             return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
                         : xlines_lds_bytes<double>(K[0], D.xC, xgen);
-        case ST_XFUSED:  // the block reduction reuses the LDS
-            return std::max(kPartialScratch, fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
-                                                  : xlines_lds_bytes<double>(K[0], D.xC, xgen));
+        case ST_XFUSED:  // the lines, the wavefront sums, the closed form's x factors
+            return (fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
+                         : xlines_lds_bytes<double>(K[0], D.xC, xgen)) +
+                   kXWaveSums + (size_t)((K[0] + 1) & ~1) * sizeof(double);
         case ST_YZ_FWD:
         case ST_YZ_INV:
             return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
@@ -791,7 +859,7 @@ void Engine::run_(Stage s, int finish) {
     sync_desc_();
     ProfScope ps(this, stage_name(s));
     launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
-                     desc_.gntz > 1}, nb,
+                     desc_.gntz > 1, desc_.infl32}, nb,
            stage_lds(s), stream_,
            finish);
 }
@@ -806,6 +874,7 @@ void Engine::run_seq_(StageSeq seq) {
 
 // ---- public device-pointer operations -----------------------------------------------
 void Engine::spread_forward(int n, const double *w_d, const double *x_d) {
+    check_staging(n, {w_d, x_d});
     ensure_atoms(n);
     set_atoms_(n, w_d, x_d, desc_.dx, desc_.phi, desc_.out);
     run_seq_(sfac_stages(slab()));
@@ -829,6 +898,7 @@ void Engine::convolve(double *out_d) {
 
 void Engine::gather(int n, const double *w_d, const double *x_d, double *dx_d, double *phi_d) {
     if (state != 1) throw UsageError("You must call de1 before this routine.");
+    check_staging(n, {w_d, x_d, dx_d, phi_d});
     ensure_atoms(n);
     set_atoms_(n, w_d, x_d, dx_d, phi_d, desc_.out);
     run_(ST_GATHER, 0);
@@ -837,6 +907,7 @@ void Engine::gather(int n, const double *w_d, const double *x_d, double *dx_d, d
 void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d,
                       double *dx_d, double *phi_d) {
     if (!has_table_) throw UsageError("No convolution kernel has been set.");
+    check_staging(n, {w_d, x_d, dx_d, phi_d});
     ensure_atoms(n);
     set_atoms_(n, w_d, x_d, dx_d, phi_d, out_d);
     const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.recs};
@@ -844,9 +915,12 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
     if (graph) {
         const bool same = graph_exec_ && graph_n_ == n && graph_slab_ == slab() &&
                           memcmp(key, graph_key_, sizeof(key)) == 0;
+        // the descriptor upload stays outside the graph, and runs before every
+        // replay: the split entry points (sfac/en/de1/de2/potl) upload their
+        // own descriptors between evaluations (a no-op when nothing changed)
+        sync_desc_();
         if (!same) {
             drop_graph_();
-            sync_desc_();  // the descriptor upload stays outside the graph
             HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             try {
                 run_seq_(eval_stages(slab(), gtile()));
@@ -993,6 +1067,7 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(gmz() & 0xffff) << 32;
     v |= (unsigned long long)(desc_.ntz > 1 ? 1 : 0) << 48;
     v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
+    v |= (unsigned long long)(desc_.infl32 ? 1 : 0) << 50;
     return v;
 }
 
@@ -1094,6 +1169,7 @@ void Engine::apply_tune(const TuneCfg &c) {
     D.gtz = c.gtz ? c.gtz : K[2];
     D.gntz = (K[2] + D.gtz - 1) / D.gtz;
     use_slab = c.slab != 0;
+    ++gen_;
     drop_graph_();
 }
 
@@ -1330,6 +1406,12 @@ void Engine::accum_cross(const Engine &b, double *acc_d) {
         accum_cross_kernel<double><<<nb, kBlock, 0, stream_>>>(
             (const double2 *)grid_, (const double2 *)b.grid_, h, acc_d);
     HIPC(hipGetLastError());
+    // and b's next work (the spread of its next frame overwrites its grid)
+    // waits for this read
+    HIPC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPC(hipEventRecord(ev, stream_));
+    HIPC(hipStreamWaitEvent(b.stream_, ev, 0));
+    HIPC(hipEventDestroy(ev));
 }
 
 void Engine::radial(const double *val_d, int nbins, double dk, double *sum_d, double *cnt_d) {
@@ -1499,31 +1581,24 @@ void Batch::rebind(const std::vector<Engine *> &frames) {
     if (frames.empty()) throw std::invalid_argument("empty batch");
     const Engine *e0 = frames[0];
     for (const Engine *e : frames) {
-        if (e->fp32 != e0->fp32 || e->fixed != e0->fixed || e->order != e0->order)
+        if (e->fp32 != e0->fp32 || e->fixed != e0->fixed || e->order != e0->order ||
+            e->desc().infl32 != e0->desc().infl32)
             throw std::invalid_argument(
-                "a batch needs one precision, accumulation mode and spline order");
+                "a batch needs one precision, accumulation mode, influence precision and "
+                "spline order");
         if (!e->has_table()) throw UsageError("No convolution kernel has been set.");
     }
     frames_ = frames;
     const int nf = (int)frames_.size();
     descs_.resize(nf);
-    for (int f = 0; f < nf; ++f) descs_[f] = frames_[f]->desc();
-    slab_ = true;
-    gtile_ = true;
-    gmz_ = 1;
-    slab_ept_ = 0;
-    xpath_ = 0;
-    zchunk_ = 0;
-    gzchunk_ = 0;
-    for (const Engine *e : frames_) {
-        zchunk_ = zchunk_ || e->desc().ntz > 1;
-        gzchunk_ = gzchunk_ || e->desc().gntz > 1;
-        slab_ = slab_ && e->slab();
-        gtile_ = gtile_ && e->gtile();
-        gmz_ = std::max(gmz_, e->gmz());
-        slab_ept_ = std::max(slab_ept_, e->slab_ept());
-        xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
+    atoms_.resize(nf);
+    gens_.resize(nf);
+    for (int f = 0; f < nf; ++f) {
+        const FrameDesc &D = frames_[f]->desc();
+        atoms_[f] = Atoms{D.n, D.w, D.x, D.out, D.dx, D.phi};
+        apply_atoms_(f);
     }
+    refresh_variant_();
     if (nf > cap_) {
         if (d_descs_) HIPC(hipFree(d_descs_));
         if (d_offs_) HIPC(hipFree(d_offs_));
@@ -1540,6 +1615,40 @@ void Batch::rebind(const std::vector<Engine *> &frames) {
     dirty_ = true;
 }
 
+void Batch::apply_atoms_(int f) {
+    Engine *e = frames_[f];
+    FrameDesc D = e->desc();
+    const Atoms &a = atoms_[f];
+    D.n = a.n;
+    D.w = a.w;
+    D.x = a.x;
+    D.out = a.out;
+    D.dx = a.dx;
+    D.phi = a.phi;
+    descs_[f] = D;
+    gens_[f] = e->desc_gen();
+}
+
+void Batch::refresh_variant_() {
+    infl32_ = frames_[0]->desc().infl32;
+    slab_ = true;
+    gtile_ = true;
+    gmz_ = 1;
+    slab_ept_ = 0;
+    xpath_ = 0;
+    zchunk_ = 0;
+    gzchunk_ = 0;
+    for (const Engine *e : frames_) {
+        zchunk_ = zchunk_ || e->desc().ntz > 1;
+        gzchunk_ = gzchunk_ || e->desc().gntz > 1;
+        slab_ = slab_ && e->slab();
+        gtile_ = gtile_ && e->gtile();
+        gmz_ = std::max(gmz_, e->gmz());
+        slab_ept_ = std::max(slab_ept_, e->slab_ept());
+        xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
+    }
+}
+
 Batch::~Batch() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
@@ -1551,7 +1660,10 @@ void Batch::set_atoms(int f, int n, const double *w_d, const double *x_d, double
                       double *dx_d, double *phi_d) {
     if (f < 0 || f >= (int)frames_.size()) throw std::invalid_argument("frame index");
     Engine *e = frames_[f];
+    e->check_staging(n, {w_d, x_d, dx_d, phi_d});
     e->ensure_atoms(n);
+    // the engine's descriptor keeps the pointers too (a later rebind of the
+    // same engines starts from them)
     FrameDesc &D = e->desc_mut();
     D.n = n;
     D.w = w_d;
@@ -1559,7 +1671,8 @@ void Batch::set_atoms(int f, int n, const double *w_d, const double *x_d, double
     D.out = out_d;
     D.dx = dx_d;
     D.phi = phi_d;
-    descs_[f] = D;
+    atoms_[f] = Atoms{n, w_d, x_d, out_d, dx_d, phi_d};
+    apply_atoms_(f);
     dirty_ = true;
 }
 
@@ -1592,6 +1705,19 @@ void Batch::upload_(hipStream_t st) {
 
 void Batch::evaluate(hipStream_t st) {
     if (!st) st = frames_[0]->stream();  // the legacy null stream cannot be captured
+    // a frame whose engine changed its box, table, launch shapes or workspace
+    // since the snapshot (set_L, set_A, autotuning, growth) is taken anew
+    bool changed = false;
+    for (int f = 0; f < (int)frames_.size(); ++f)
+        if (frames_[f]->desc_gen() != gens_[f]) {
+            if (!frames_[f]->has_table()) throw UsageError("No convolution kernel has been set.");
+            apply_atoms_(f);
+            changed = true;
+        }
+    if (changed) {
+        refresh_variant_();
+        dirty_ = true;
+    }
     if (dirty_) upload_(st);
     const int nf = (int)frames_.size();
     Engine *e0 = frames_[0];
@@ -1601,7 +1727,7 @@ void Batch::evaluate(hipStream_t st) {
         for (int i = 0; i < seq.n; ++i) {
             const Stage s = seq.s[i];
             const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_,
-                           zchunk_, gzchunk_};
+                           zchunk_, gzchunk_, infl32_};
             e0->prof_begin_on(stage_name(s), st);
             e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
             e0->prof_end_on(st);

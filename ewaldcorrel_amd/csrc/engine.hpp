@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <array>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -18,6 +19,7 @@ namespace spme {
 enum EngineFlags {
     kFlagFP32 = 1,        // spreading, FFTs and gather in FP32
     kFlagFixedPoint = 2,  // deterministic int64 fixed-point spreading (always on for FP32)
+    kFlagInfl32 = 4,      // FP32 engine: closed-form influence and thread sums in FP32
 };
 
 // Kernel stages.  An evaluation runs PREP..SPREAD, YZ_FWD, XFUSED, YZ_INV,
@@ -51,6 +53,7 @@ struct Launch {
     int gmz;       // row chunks of 64 per lane of the tiled gather (ceil((K2 + n - 1) / 64))
     int zchunk;    // some frame spreads through z-chunked tiles
     int gzchunk;   // some frame gathers through z-chunked tiles
+    int infl32;    // the frames use the FP32 influence option (FP32 engine)
 };
 
 struct FftPlan {
@@ -120,6 +123,14 @@ public:
     double *dx_dev() const { return dx_d_; }
     double *phi_dev() const { return phi_d_; }
     size_t atom_capacity() const { return atom_cap_; }
+    // Descriptor generation: bumped whenever the box, the influence table,
+    // the launch shapes or the workspace buffers change, so that a batch
+    // holding a snapshot of the descriptor knows to take a new one.
+    unsigned long long desc_gen() const { return gen_; }
+    // std::invalid_argument if one of ptrs is this engine's own atom staging
+    // buffer (x_dev, w_dev, dx_dev, phi_dev) and n exceeds its capacity: the
+    // growth that n needs would release the buffer the caller points at
+    void check_staging(int n, std::initializer_list<const void *> ptrs) const;
     // kernel variants an evaluation of this engine launches (precision,
     // accumulation, order, transform and gather paths, slab/gather template
     // sizes); a batch of engines with one signature runs each frame through
@@ -182,6 +193,10 @@ public:
     const FrameDesc &desc() const { return desc_; }
     FrameDesc &desc_mut() { return desc_; }
     int stage_blocks(Stage s) const;
+    int stage_blocks_n(Stage s, int natoms) const;  // for natoms atoms
+    // per-atom sort workspace: bytes for n atoms, and its parts in D at base
+    size_t workspace_bytes(size_t n) const;
+    void bind_workspace(FrameDesc &D, void *base, size_t n) const;
     // xpath: the x path of the launch (kXBlock, or kXGeneric for a batch with
     // a generic frame); the frame's own path when that is more general
     size_t stage_lds(Stage s, int xpath = 0) const;
@@ -214,6 +229,7 @@ public:
 
 private:
     void alloc_(void **p, size_t bytes);
+    void free_(void *p);  // release one alloc_ buffer (stream synchronised)
     void run_(Stage s, int finish);  // single-frame launch of one stage on stream_
     void run_seq_(StageSeq seq);
     bool graph_slab_ = false;        // path the captured graph was recorded with
@@ -236,6 +252,7 @@ private:
     double *result_ = nullptr;  // 8 doubles
     int nred_infl_ = 1;
     double *x_d_ = nullptr, *w_d_ = nullptr, *dx_d_ = nullptr, *phi_d_ = nullptr;
+    void *ws_ = nullptr;  // per-atom sort workspace (bind_workspace)
     size_t atom_cap_ = 0;
     size_t work_cap_ = 0;
     double *pin_ = nullptr;  // pinned staging
@@ -253,7 +270,8 @@ private:
     SplineArgs<float> sp_f_;
     SplineArgs<double> sp_d_;
     size_t dev_bytes_ = 0;
-    std::vector<void *> allocs_;
+    std::vector<std::pair<void *, size_t>> allocs_;
+    unsigned long long gen_ = 1;
 
     // captured evaluation
     hipGraphExec_t graph_exec_ = nullptr;
@@ -292,9 +310,20 @@ public:
     bool use_graph = true;
 
 private:
+    // per-frame atom pointers given by set_atoms (kept apart from the
+    // engines' descriptors, which an engine's own calls overwrite)
+    struct Atoms {
+        int n;
+        const double *w, *x;
+        double *out, *dx, *phi;
+    };
     void upload_(hipStream_t st);
+    void refresh_variant_();  // launch variant flags from the frames
+    void apply_atoms_(int f);  // descs_[f] = engine descriptor + atoms_[f]
     std::vector<Engine *> frames_;
     std::vector<FrameDesc> descs_;
+    std::vector<Atoms> atoms_;
+    std::vector<unsigned long long> gens_;  // engine descriptor generation of descs_
     FrameDesc *d_descs_ = nullptr;
     int *d_offs_ = nullptr;                 // ST_COUNT x (nf + 1)
     int cap_ = 0;                           // frames the device arrays hold
@@ -309,6 +338,7 @@ private:
     int xpath_ = 0;                         // x transform path valid for every frame
     int zchunk_ = 0;                        // some frame spreads through z-chunked tiles
     int gzchunk_ = 0;                       // some frame gathers through z-chunked tiles
+    int infl32_ = 0;                        // the frames use the FP32 influence option
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

@@ -180,6 +180,46 @@ __device__ __forceinline__ float xline_point_acc(const XLineF32 &L, const FrameD
     return a;
 }
 
+// e^x for x <= 0 in FP64 with few registers: Cody-Waite reduction
+// x = k ln2 + r, |r| <= ln2/2, with ln2 split into a 32-bit head (k ln2_hi
+// exact for |k| < 2^21) and its remainder, then the Taylor series of e^r to
+// degree 13 (truncation below 2e-17 relative) by Horner, scaled by 2^k.  A
+// result below the FP64 range is flushed to zero.
+__device__ __forceinline__ double exp_nonpos(double x) {
+    constexpr double kLn2Hi = 0.693147180369123816490;  // ln 2 to 32 bits
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;
+    constexpr double kInvLn2 = 1.44269504088896338700;
+    const double k = rint(x * kInvLn2);
+    const double r = (x - k * kLn2Hi) - k * kLn2Lo;
+    double p = 1.0 / 6227020800.0;  // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return k < -1074.0 ? 0.0 : ldexp(p, (int)k);
+}
+
+// m of image (ix, iy, iz) in ewald_image's arithmetic (same products and sums,
+// no contraction), so a ball test on it decides as the table builder does
+__device__ __forceinline__ void xline64_m(const CellArgs &c, int ix, int iy, int iz, double &mx,
+                                          double &my, double &mz, double &m2) {
+#pragma clang fp contract(off)
+    const double nx = ix, ny = iy, nz = iz;
+    mx = nx * c.iL00;
+    my = nx * c.iL10 + ny * c.iL11;
+    mz = nx * c.iL20 + ny * c.iL21 + nz * c.iL22;
+    m2 = mx * mx + my * my + mz * mz;
+}
+
 __device__ __forceinline__ void table_point(const FrameDesc &F, long h, int kx, int ky, int kz,
                                             double &A, double *dA) {
     if (F.tabmode == kTableAnalytic) {
@@ -407,28 +447,29 @@ __host__ __device__ inline int x_path(const FrameDesc &F) {
     return F.plan0.generic ? kXGeneric : kXBlock;
 }
 
-// Thread sums of the fused x pass: FP32 in the FP32 engine (seven registers
-// fewer held through the inverse transform), FP64 otherwise.
-template <typename T>
-using XAcc = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+// Wavefront sums of the fused x pass, parked in LDS after the lines (one row
+// of 8 doubles per wavefront) between the influence step and the inverse
+// transform, so that no sums are held in registers through the inverse.
+constexpr size_t kXWaveSums = (kBlock / 64) * 8 * sizeof(double);
 
 // The influence step of the fused x pass on the transformed lines in LDS:
 // S-hat = S iB0 iB1 iB2, the energy and virial sums of A |S-hat|^2 (thread
 // sums in acc[7]) and S-hat A iB (x2 on the kz = 0 and Nyquist planes, which
 // stand for their Hermitian partners) written back for the inverse.  Thread
 // (c, e0) holds column f = b C + c (ky, kz) and elements e0 + q es.
-template <typename T>
+template <typename T, int INF>
 __device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
                                             const double *ib0, int LS, int c, int e0, int es,
                                             int f, bool valid, int ky, int kz, double ibyz,
-                                            XAcc<T> acc[7]) {
+                                            double acc[7]) {
     constexpr int EPT = kXElems / kBlock;
     const int N = F.plan0.N;
     const bool even = (F.K2 % 2) == 0;
     const double two = (kz == 0 || (even && kz == F.ldim - 1)) ? 2.0 : 1.0;
-    if (std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
-        // FP32 engine, closed form: FP32 terms, each thread's sums (at most
-        // kXElems / kBlock points) in FP32, the block sums in FP64.  With a
+    if (INF && std::is_same<T, float>::value && F.tabmode == kTableAnalytic) {
+        // FP32 engine with the FP32 influence option, closed form: FP32 terms,
+        // each thread's sums (at most kXElems / kBlock points) in FP32, the
+        // wavefront and block sums in FP64.  With a
         // the sum of f = exp(-pi^2 m^2 / beta^2) / (pi m^2) over the point's
         // images: A = mult a / 2 and dA = mult sum fp m m^T, and since
         // mult x two = 2 on every plane, the output factor A iB two is a iB;
@@ -453,15 +494,19 @@ __device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
                 res[c * LS + e] = v;
             }
         }
-        acc[0] = XAcc<T>(0.5f * L.mult * aE);
-        acc[1] = XAcc<T>(L.mult * d0);
-        acc[2] = XAcc<T>(L.mult * d1);
-        acc[3] = XAcc<T>(L.mult * d2);
-        acc[4] = XAcc<T>(L.mult * d3);
-        acc[5] = XAcc<T>(L.mult * d4);
-        acc[6] = XAcc<T>(L.mult * d5);
-    } else {  // FP64 terms and thread sums (the FP32 engine rounds its thread sums)
-        double accd[7] = {0, 0, 0, 0, 0, 0, 0};
+        acc[0] = (double)(0.5f * L.mult * aE);
+        acc[1] = (double)(L.mult * d0);
+        acc[2] = (double)(L.mult * d1);
+        acc[3] = (double)(L.mult * d2);
+        acc[4] = (double)(L.mult * d3);
+        acc[5] = (double)(L.mult * d4);
+        acc[6] = (double)(L.mult * d5);
+    } else {
+        // FP64 /B, table influence and thread sums (each table entry added
+        // to the sums as it is read)
+        const long H = (long)F.K0 * F.K1 * F.ldim;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) acc[k] = 0.0;
 #pragma unroll 1
         for (int q = 0; q < EPT; ++q) {
             const int e = e0 + q * es;
@@ -470,32 +515,196 @@ __device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
                 cplx<T> v = res[c * LS + e];
                 const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
                 const double s = re * re + im * im;
-                double A, dA[6];
-                table_point(F, (long)e * F.K1 * F.ldim + f, e, ky, kz, A, dA);
-                accd[0] += A * s;
+                const long h = (long)e * F.K1 * F.ldim + f;
+                const double A = F.table[h];
+                const double *d = F.table + H + 6 * h;
+                acc[0] += A * s;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) accd[1 + k] += s * dA[k];
+                for (int k = 0; k < 6; ++k) acc[1 + k] += s * d[k];
                 const double tt = A * ib * two;
                 v.x = T(re * tt);
                 v.y = T(im * tt);
                 res[c * LS + e] = v;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) acc[k] = XAcc<T>(accd[k]);
     }
 }
 
-// waves per SIMD the fused x launches are held to (registers): FP32 at 64
-// VGPRs (line transforms with per-stage indices) runs eight workgroups per
-// CU (FP64 at 102 would spill)
+// FP64 closed-form influence of the thread's x line (its column: fixed ky,
+// kz) in the fused x pass, in both engines (the FP32 engine rounds only the
+// line values it writes back): /B, A, the energy and the virial in FP64.
+//   * An image of point kx is m = nx a + sy u + sz z with a = (iL00, iL10,
+//     iL20), u = ny (0, iL11, iL21), z = nz (0, 0, iL22) and signs sy, sz = +1
+//     except on the y and z Nyquist lines (+-1 images there).  Its virial
+//     term w m_a m_b (w = mult s fp) expands into five sums over the line,
+//     W2 = sum w nx^2, Wy = sum w nx sy, Wz = sum w nx sz, W0 = sum w,
+//     Wyz = sum w sy sz, folded once per thread:
+//       v_ab = W2 a_a a_b + Wy (a_a u_b + a_b u_a) + Wz (a_a z_b + a_b z_a)
+//              + W0 (u_a u_b + z_a z_b) + Wyz (u_a z_b + z_a u_b).
+//   * Orthorhombic boxes (iL10 = iL20 = 0): e^(fac m^2) = gx[kx] e^(fac |b|^2)
+//     (b = sy u + sz z), gx[kx] = e^(fac iL00^2 nx^2) formed once per
+//     workgroup in LDS; other boxes take e^(fac m^2) per image.  1/m^2 by the
+//     hardware reciprocal and two Newton steps.
+//   * m^2 is formed with contractions; within a relative 1e-12 of the cutoff
+//     the ball test is redone exactly as ewald_image computes m, so the set
+//     of images is the table's.
+// The wavefront sums of E and the virial go to wsums[wavefront][0..6].
+__device__ __forceinline__ double recip_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+template <typename T>
+__device__ __forceinline__ void x_influence64(const FrameDesc &F, cplx<T> *res,
+                                              const double *ib0, const double *gx, int LS,
+                                              int c, int e0, int es, bool valid, int ky, int kz,
+                                              double ibyz, double *wsums) {
+    constexpr int EPT = kXElems / kBlock;
+    const int N = F.plan0.N, K0 = F.K0, K1 = F.K1, K2 = F.K2;
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac, max_m2 = F.max_m2;
+    const bool ortho = cl.iL10 == 0.0 && cl.iL20 == 0.0;
+    const double q00 = cl.iL00 * cl.iL00 + cl.iL10 * cl.iL10 + cl.iL20 * cl.iL20;
+    const int ny = ky <= K1 / 2 ? ky : ky - K1;
+    const bool yny = (K1 % 2) == 0 && ky == K1 / 2;
+    const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
+    const bool zn2 = zny && kz > 0;
+    const double mult = (kz == 0 || zny) ? 1.0 : 2.0;
+    // b = sy u + sz z of image class (sy, sz): its y, z components, |b|^2, a.b
+    auto bvec = [&](int sy, int sz, double &B1, double &B2) {
+        const double nyd = (double)(sy * ny), nzd = (double)(sz * kz);
+        B1 = nyd * cl.iL11;
+        B2 = fma(nyd, cl.iL21, nzd * cl.iL22);
+    };
+    // the line's own class (+, +)
+    double c0, c1, el;
+    {
+        double B1, B2;
+        bvec(1, 1, B1, B2);
+        c0 = fma(B1, B1, B2 * B2);
+        c1 = fma(cl.iL10, B1, cl.iL20 * B2);
+        el = ortho ? exp_nonpos(fac * c0) * (1.0 / M_PI) : 0.0;
+    }
+    double W0 = 0.0, W2 = 0.0, Wy = 0.0, Wz = 0.0, Wyz = 0.0, aE = 0.0;
+    // m^2 = q00 nx^2 + 2 nx c1 + c0 of image nx (class constants c0, c1) and
+    // its ball test (redone exactly near the cutoff)
+    auto in_ball = [&](int nx, int sy, int sz, double cc0, double cc1, double &m2) {
+        const double nxd = nx;
+        m2 = fma(fma(q00, nxd, 2.0 * cc1), nxd, cc0);
+        bool in = !(m2 < 1e-10 || m2 > max_m2);
+        if (fabs(m2 - max_m2) <= 1e-12 * max_m2) {
+            double mx, my, mz, m2e;
+            xline64_m(cl, nx, sy * ny, sz * kz, mx, my, mz, m2e);
+            in = !(m2e < 1e-10 || m2e > max_m2);
+        }
+        return in;
+    };
+    // the terms of one image: f returned, moments added
+    auto add_image = [&](int nx, int sy, int sz, double m2, double ex, double s) {
+        const double r = recip_f64(m2);
+        const double f = ex * r;
+        const double w = mult * s * ((fac - r) * f);
+        const double nxd = nx;
+        const double t = w * nxd;
+        W0 += w;
+        W2 = fma(t, nxd, W2);
+        Wy = sy > 0 ? Wy + t : Wy - t;
+        Wz = sz > 0 ? Wz + t : Wz - t;
+        Wyz = sy * sz > 0 ? Wyz + w : Wyz - w;
+        return f;
+    };
+    const bool nyq_line = yny || zn2;
+#pragma unroll 1
+    for (int q = 0; q < EPT; ++q) {
+        const int e = e0 + q * es;
+        if (e >= N || !valid) continue;
+        const double ib = ib0[e] * ibyz;
+        cplx<T> v = res[c * LS + e];
+        const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+        const double s = re * re + im * im;
+        const int nx = e <= K0 / 2 ? e : e - K0;
+        const bool xny = (K0 % 2) == 0 && e == K0 / 2;
+        double a = 0.0;
+        if (!(xny || nyq_line)) {
+            // one image, the line's own class
+            double m2;
+            if (in_ball(nx, 1, 1, c0, c1, m2))
+                a = add_image(nx, 1, 1, m2,
+                              ortho ? gx[e] * el : exp_nonpos(m2 * fac) * (1.0 / M_PI), s);
+        } else {
+            // Nyquist planes: every signed image, its class constants anew
+            const int nimg = (xny ? 2 : 1) * (yny ? 2 : 1) * (zn2 ? 2 : 1);
+#pragma unroll 1
+            for (int k = 0; k < nimg; ++k) {
+                int kk = k, nxs = nx, sy = 1, sz = 1;
+                if (xny) {
+                    if (kk & 1) nxs = -nx;
+                    kk >>= 1;
+                }
+                if (yny) {
+                    if (kk & 1) sy = -1;
+                    kk >>= 1;
+                }
+                if (zn2 && (kk & 1)) sz = -1;
+                double B1, B2, m2;
+                bvec(sy, sz, B1, B2);
+                const double cc0 = fma(B1, B1, B2 * B2);
+                const double cc1 = fma(cl.iL10, B1, cl.iL20 * B2);
+                if (!in_ball(nxs, sy, sz, cc0, cc1, m2)) continue;
+                const double ex = ortho ? gx[e] * exp_nonpos(fac * cc0) * (1.0 / M_PI)
+                                        : exp_nonpos(m2 * fac) * (1.0 / M_PI);
+                a += add_image(nxs, sy, sz, m2, ex, s);
+            }
+        }
+        aE = fma(a, s, aE);
+        const double tt = a * ib;
+        v.x = T(re * tt);
+        v.y = T(im * tt);
+        res[c * LS + e] = v;
+    }
+    // fold: a = (iL00, iL10, iL20), u = (0, uy, uz), z = (0, 0, zz)
+    const double uy = (double)ny * cl.iL11, uz = (double)ny * cl.iL21;
+    const double zz = (double)kz * cl.iL22;
+    const double a0 = cl.iL00, a1 = cl.iL10, a2 = cl.iL20;
+    double vv[7];
+    vv[0] = 0.5 * mult * aE;
+    vv[1] = W2 * a0 * a0;                                                // xx
+    vv[2] = W2 * a1 * a1 + 2.0 * Wy * a1 * uy + W0 * uy * uy;            // yy
+    vv[3] = W2 * a2 * a2 + 2.0 * (Wy * a2 * uz + Wz * a2 * zz) +
+            W0 * (uz * uz + zz * zz) + 2.0 * Wyz * uz * zz;              // zz
+    vv[4] = W2 * a1 * a0 + Wy * a0 * uy;                                 // yx
+    vv[5] = W2 * a2 * a0 + Wy * a0 * uz + Wz * a0 * zz;                  // zx
+    vv[6] = W2 * a2 * a1 + Wy * (a2 * uy + a1 * uz) + Wz * a1 * zz +
+            W0 * uy * uz + Wyz * uy * zz;                                // zy
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const double vs = wave_sum(vv[k]);
+        if (lane == 0) wsums[wv * 8 + k] = vs;
+    }
+}
+
+// The influence of the fused x pass is a template choice (INF): 0 the FP64
+// forms (table or closed form), 1 the FP32 closed form of the FP32 engine's
+// option (table frames still take the FP64 table path).  Each is its own
+// instantiation, so that neither holds the other's registers.
+// Waves per SIMD the fused x launches are held to (registers): FP32 with the
+// FP32 influence at 64 VGPRs (line transforms with per-stage indices) runs
+// eight workgroups per CU; with the FP64 influence five (its FP64 per-point
+// terms need ~96 VGPRs: at six or more they spill inside the point loop, and
+// five and four measured the same, 141 us per 16-frame water launch, so the
+// pass is bound by its FP64 arithmetic there); FP64 at 102 would spill.
+constexpr int kXWaves64 = 5;
 template <typename T, int MODE> constexpr bool x_lean() {
     return MODE == 2 && std::is_same<T, float>::value;
 }
-template <typename T, int MODE> constexpr int x_waves() { return x_lean<T, MODE>() ? 8 : 1; }
+template <typename T, int MODE, int INF> constexpr int x_waves() {
+    return x_lean<T, MODE>() ? (INF ? 8 : kXWaves64) : 1;
+}
 
-template <typename T, int MODE, int XP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE>()))))
+template <typename T, int MODE, int XP, int INF>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE, INF>()))))
     fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int EPT = kXElems / kBlock;
@@ -530,6 +739,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
         load_twiddles<T>(twl, F.tw0, N);
         if (MODE != 0)
             for (int e = threadIdx.x; e < N; e += kBlock) ib0[e] = F.iB0[e];
+        if (MODE == 2 && !INF && F.tabmode == kTableAnalytic && F.tcell.iL10 == 0.0 &&
+            F.tcell.iL20 == 0.0) {
+            // gx[kx] = e^(fac iL00^2 nx^2), after the lines and the wavefront sums
+            double *gx = reinterpret_cast<double *>(smem_raw + xlines_lds_bytes<T>(N, C, XP == kXGeneric) +
+                                                    kXWaveSums);
+            const double q00 = F.tcell.iL00 * F.tcell.iL00;
+            for (int e = threadIdx.x; e < N; e += kBlock) {
+                const double nx = e <= N / 2 ? e : e - N;
+                gx[e] = exp_nonpos(F.ewald_fac * q00 * nx * nx);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = e0 + q * es;
@@ -551,9 +771,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
         const bool valid = f < P;
         const int fv = valid ? f : P - 1;
         const int ky = fdiv(fv, F.dldim), kz = fv - ky * ldim;
-        XAcc<T> acc[7];
-        x_influence<T>(F, res, ib0, LS, c, e0, es, f, valid, ky, kz,
-                       F.iB1[ky] * F.iB2[kz], acc);
+        double *wsums = reinterpret_cast<double *>(
+            smem_raw + xlines_lds_bytes<T>(N, C, XP == kXGeneric));
+        if (!INF && F.tabmode == kTableAnalytic) {
+            x_influence64<T>(F, res, ib0, wsums + 8 * (kBlock / 64), LS, c, e0, es, valid, ky,
+                             kz, F.iB1[ky] * F.iB2[kz], wsums);
+        } else {
+            double acc[7];
+            x_influence<T, INF>(F, res, ib0, LS, c, e0, es, f, valid, ky, kz,
+                                F.iB1[ky] * F.iB2[kz], acc);
+            // wavefront sums (a fixed butterfly order) parked in LDS
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const double v = wave_sum(acc[k]);
+                if (lane == 0) wsums[wv * 8 + k] = v;
+            }
+        }
         __syncthreads();
         res = x_transform<T, XP == kXGeneric, x_lean<T, MODE>()>(res, res == lines ? lines2 : lines, C, dC, LS, p,
                                               twl, +1);
@@ -568,10 +802,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
             const int e = es0 + q * es;
             if (e < N && vs) cols[(long)e * P] = res[cs * LS + e];
         }
-        double accd[7];
+        // the workgroup's partial: its wavefront sums in order (the inverse
+        // transform's barriers made them visible)
+        if (threadIdx.x < 7) {
+            double v = 0.0;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) accd[k] = (double)acc[k];
-        block_partial(F, b, accd, reinterpret_cast<double *>(smem_raw));  // LDS >= kPartialScratch
+            for (int q = 0; q < kBlock / 64; ++q) v += wsums[q * 8 + threadIdx.x];
+            F.partials[b * 8 + threadIdx.x] = v;
+        }
         return;
     }
     __syncthreads();

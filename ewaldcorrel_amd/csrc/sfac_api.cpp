@@ -325,7 +325,8 @@ int sfac_info(void *h, long long out[12]) {
         out[2] = e->K[2];
         out[3] = e->ldim;
         out[4] = e->order;
-        out[5] = (e->fp32 ? SFAC_FP32 : 0) | (e->fixed ? SFAC_FIXED : 0);
+        out[5] = (e->fp32 ? SFAC_FP32 : 0) | (e->fixed ? SFAC_FIXED : 0) |
+                 (e->desc().infl32 ? SFAC_FP32_INFLUENCE : 0);
         out[6] = (long long)e->device_bytes();
         out[7] = e->state;
         out[8] = e->table_mode();

@@ -10,7 +10,7 @@ phi)``; module-level ``eta``, ``ewald_f``, ``LofS``, ``SofL``, ``array``.
 the HIP library, and there is no fallback when it is missing.
 
 Additions (keyword arguments and new methods only):
-  * ``Sfac(..., precision="fp64"|"fp32", fixed=False)``
+  * ``Sfac(..., precision="fp64"|"fp32", fixed=False, fp32_influence=False)``
   * ``set_A`` recognises the closed-form Ewald kernel (``ewald_f`` or a
     kernel from ``ewald_kernel(eta)``) and builds the table natively,
     instead of one Python callback per lattice point
@@ -33,6 +33,7 @@ array = np.array
 SFAC_FP64 = 0
 SFAC_FP32 = 1
 SFAC_FIXED = 2
+SFAC_FP32_INFLUENCE = 4
 
 _LIB = None
 # double (*rad_fn)(double r2, double *dA, void *info), called with the GIL held
@@ -157,7 +158,7 @@ def _check(rc, what):
 class Sfac:
     """Handle on one SPME system (ewald.py:70-122)."""
 
-    def __init__(self, L, K, order=4, precision="fp64", fixed=False):
+    def __init__(self, L, K, order=4, precision="fp64", fixed=False, fp32_influence=False):
         sfac_p, rad_fn, libsfac = describe_sfac()
         self._libsfac = libsfac
         self.rad_fn = rad_fn
@@ -171,6 +172,10 @@ class Sfac:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         if fixed:
             flags |= SFAC_FIXED
+        if fp32_influence:
+            if precision != "fp32":
+                raise ValueError("fp32_influence applies to precision='fp32' only")
+            flags |= SFAC_FP32_INFLUENCE
         self.precision = precision
         L = np.ascontiguousarray(np.asarray(L, dtype=np.float64))
         self._L6 = L.copy()

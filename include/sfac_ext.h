@@ -20,6 +20,10 @@ extern "C" {
 #define SFAC_FP32 1  /* spread/FFT/gather FP32; table, reductions FP64; int64   */
                      /* fixed-point grid accumulation                          */
 #define SFAC_FIXED 2 /* deterministic int64 fixed-point spreading (FP64 path)   */
+#define SFAC_FP32_INFLUENCE 4 /* with SFAC_FP32: the closed-form Ewald influence   */
+                              /* and the per-thread energy/virial sums in FP32    */
+                              /* (faster; by default they are FP64, as are /B,    */
+                              /* the table and every reduction)                   */
 
 #define SFAC_ERR_USAGE (-1)  /* call out of order (reference prints an error) */
 #define SFAC_ERR_ARG (-2)    /* invalid argument                                */

@@ -224,3 +224,32 @@ def test_batch_rejects_mixed_precision():
     b.set_A(ew().ewald_kernel(0.4), 1.0)
     with pytest.raises(RuntimeError, match="precision"):
         ew().SfacBatch([a, b])
+
+
+def test_fp32_influence_option():
+    """The FP32 engine evaluates /B, the influence and the energy/virial sums
+    in FP64 by default; fp32_influence evaluates the closed form and the
+    thread sums in FP32 (its own kernel variant).  Both meet the FP32 bars
+    against the FP64 oracle, the default is the closer of the two, and a
+    batch refuses to mix them."""
+    from ewaldcorrel_amd.workloads import water
+    wsy = water(2048, 40.0, (48, 48, 48), seed=9)
+    o = OracleSfac(wsy.L6, wsy.K, 4)
+    o.set_A(ewald_rad(wsy.beta), 1.0)
+    E0, v0, dx0, phi0 = o.evaluate(wsy.w, wsy.x)
+    errs = {}
+    hs = {}
+    for opt in (False, True):
+        s = ew().Sfac(wsy.L6, np.array(wsy.K), 4, precision="fp32", fp32_influence=opt)
+        s.set_A(ew().ewald_kernel(wsy.beta), 1.0)
+        assert s.info()["table_mode"] == 1
+        assert bool(s.info()["flags"] & ew().SFAC_FP32_INFLUENCE) == opt
+        E, vir, dx, phi = s.evaluate(wsy.w, wsy.x)
+        errs[opt] = (abs(E - E0) / abs(E0), rel(vir, v0), rel_rms(dx, dx0))
+        say("fp32_influence", opt, errs[opt])
+        assert errs[opt][0] <= 1e-6 and errs[opt][1] <= 1e-5 and errs[opt][2] <= 1e-5
+        hs[opt] = s
+    with pytest.raises(RuntimeError):
+        ew().SfacBatch([hs[False], hs[True]])
+    with pytest.raises(ValueError):
+        ew().Sfac(wsy.L6, np.array(wsy.K), 4, precision="fp64", fp32_influence=True)
