@@ -539,6 +539,8 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     D.nred_i = nred_infl_;
     const int nred_max = std::max(nred_infl_, D.xB);
     alloc_((void **)&d_desc_, sizeof(FrameDesc));
+    alloc_((void **)&d_desc_na_, sizeof(FrameDesc));
+    HIPC(hipGetDevice(&dev_));
     memset(&desc_uploaded_, 0xff, sizeof(FrameDesc));
     alloc_((void **)&D.partials, (size_t)nred_max * 8 * sizeof(double));
     // the scan reads the counters in zero-padded tiles of 1024 x 16
@@ -939,6 +941,72 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
         HIPC(hipGraphLaunch(graph_exec_, stream_));
     } else {
         run_seq_(eval_stages(slab(), gtile()));
+    }
+    state = 1;
+}
+
+// ---- evaluation without device allocation ------------------------------------------------
+// The per-atom workspace of an allocation-free evaluation lives in one static
+// device array per GPU (held until the process ends).  Its device address is
+// looked up once per device with that device current; host code passes that
+// address, never the array's name, to kernels and copies.  Nothing in it needs
+// to start at zero (the column counters, which must, belong to the handle and
+// are cleared by the scan of every evaluation).
+namespace {
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+void *scratch_ptr(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        HIPC(hipGetDevice(&cur));
+        if (cur != dev) HIPC(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        hip_check(e, "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void Engine::evaluate_noalloc(int n, const double *w_d, const double *x_d, double *out_d,
+                              double *dx_d, double *phi_d) {
+    if (!has_table_) throw UsageError("No convolution kernel has been set.");
+    if (n < 0) throw std::invalid_argument("n < 0");
+    const size_t need = workspace_bytes((size_t)n);
+    if (need > kNoAllocScratchBytes)
+        throw std::invalid_argument(
+            "the allocation-free evaluation holds " + std::to_string(kNoAllocScratchBytes) +
+            " bytes of scratch; n = " + std::to_string(n) + " atoms need " +
+            std::to_string(need) + " (use sfac_eval_dev)");
+    void *base = scratch_ptr(dev_);
+    desc_na_ = desc_;
+    desc_na_.n = n;
+    desc_na_.w = w_d;
+    desc_na_.x = x_d;
+    desc_na_.out = out_d;
+    desc_na_.dx = dx_d;
+    desc_na_.phi = phi_d;
+    bind_workspace(desc_na_, base, (size_t)n);
+    // pageable source: the call returns once the bytes are staged
+    HIPC(hipMemcpyAsync(d_desc_na_, &desc_na_, sizeof(FrameDesc), hipMemcpyHostToDevice,
+                        stream_));
+    const Launch L{d_desc_na_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
+                   desc_.gntz > 1, desc_.infl32};
+    const StageSeq seq = eval_stages(slab(), gtile());
+    Stage prev = ST_COUNT;
+    for (int i = 0; i < seq.n; ++i) {
+        const Stage st = seq.s[i];
+        launch(st, L, stage_blocks_n(st, n), stage_lds(st), stream_, finishes(prev, st));
+        prev = st;
     }
     state = 1;
 }

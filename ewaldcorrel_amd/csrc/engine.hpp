@@ -16,6 +16,9 @@
 
 namespace spme {
 
+// Scratch of the allocation-free evaluation: one static device array per GPU.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+
 enum EngineFlags {
     kFlagFP32 = 1,        // spreading, FFTs and gather in FP32
     kFlagFixedPoint = 2,  // deterministic int64 fixed-point spreading (always on for FP32)
@@ -100,6 +103,14 @@ public:
     // hipGraph when called again with the same arguments and use_graph
     void evaluate(int n, const double *w_d, const double *x_d, double *out_d, double *dx_d,
                   double *phi_d);
+
+    // The same evaluation with no device allocation: the per-atom workspace
+    // in the library's static scratch array of this GPU (kNoAllocScratchBytes,
+    // shared by every handle on it, so overlapping calls race), kernels
+    // launched one by one (no graph).  std::invalid_argument if n atoms need
+    // more scratch than that.
+    void evaluate_noalloc(int n, const double *w_d, const double *x_d, double *out_d,
+                          double *dx_d, double *phi_d);
 
     // host-pointer helpers (stage through pinned memory, synchronous results)
     void sfac_host(int n, const double *w, const double *x);
@@ -243,6 +254,9 @@ private:
     FrameDesc desc_;
     FrameDesc desc_uploaded_;
     FrameDesc *d_desc_ = nullptr;
+    FrameDesc desc_na_;                // descriptor of the allocation-free evaluation
+    FrameDesc *d_desc_na_ = nullptr;
+    int dev_ = 0;                      // device the engine was made on
     void *grid_ = nullptr;
     double *table_ = nullptr;  // A[H] then dA[6H]
     double *table_stats_ = nullptr;  // per-workgroup table-vs-closed-form statistics

@@ -285,6 +285,17 @@ int sfac_eval_dev(void *h, int n, const double *w_d, const double *x_d, double *
     });
 }
 
+int sfac_eval_dev_noalloc(void *h, int n, const double *w_d, const double *x_d, double *out_d,
+                          double *dx_d, double *phi_d) {
+    return guard([&] {
+        if (n < 0) throw std::invalid_argument("n < 0");
+        SFac *s = as_sfac(h);
+        spme::Engine *e = s->engine;
+        e->evaluate_noalloc(n, w_d, x_d, out_d ? out_d : e->result_dev(), dx_d, phi_d);
+        s->state = e->state;
+    });
+}
+
 double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6], double *dx,
                  double *phi) {
     double E = 0.0;

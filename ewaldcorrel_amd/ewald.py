@@ -111,6 +111,8 @@ def describe_sfac():
         int_fn(lib.potl_dev, sfac_p, c_int, c_void_p, c_void_p)
         int_fn(lib.sfac_eval_dev, sfac_p, c_int, c_void_p, c_void_p, c_void_p,
                c_void_p, c_void_p)
+        int_fn(lib.sfac_eval_dev_noalloc, sfac_p, c_int, c_void_p, c_void_p, c_void_p,
+               c_void_p, c_void_p)
         dbl_fn(lib.sfac_eval, sfac_p, c_int, nparr, nparr, nparr_t(6), nparr, nparr)
         decl_fn(lib.sfac_stream, sfac_p, c_void_p)
         int_fn(lib.sfac_sync, sfac_p)
@@ -272,6 +274,12 @@ class Sfac:
     def eval_dev(self, n, w_ptr, x_ptr, out_ptr, dx_ptr, phi_ptr):
         _check(self._libsfac.sfac_eval_dev(self._data, n, w_ptr, x_ptr, out_ptr,
                                            dx_ptr, phi_ptr), "sfac_eval_dev")
+
+    def eval_dev_noalloc(self, n, w_ptr, x_ptr, out_ptr, dx_ptr, phi_ptr):
+        """eval_dev with no device allocation (workspace in the library's
+        static scratch of this GPU; overlapping calls on one GPU race)."""
+        _check(self._libsfac.sfac_eval_dev_noalloc(self._data, n, w_ptr, x_ptr, out_ptr,
+                                                   dx_ptr, phi_ptr), "sfac_eval_dev_noalloc")
 
     def profile(self, enable=True):
         """Bracket every kernel launch with HIP events on ``stream``."""

@@ -59,6 +59,18 @@ int potl_dev(void *h, int n, const double *x_d, double *phi_d);
  * dx_d or phi_d may be NULL to skip that output. */
 int sfac_eval_dev(void *h, int n, const double *w_d, const double *x_d, double *out_d,
                   double *dx_d, double *phi_d);
+/* sfac_eval_dev with no device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  The results are those of
+ * sfac_eval_dev.  The per-atom sort workspace lives in the library's static
+ * scratch array of SFAC_NOALLOC_SCRATCH_BYTES per GPU (enough for about
+ * 1.6 M atoms in FP32 or 1.2 M in FP64), shared by every handle on that GPU:
+ * calls that overlap on one GPU race.  Kernels are launched one by one (no
+ * graph capture or instantiation).  Returns SFAC_ERR_ARG, with the limit in
+ * the message, when n atoms need more scratch than that. */
+#define SFAC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sfac_eval_dev_noalloc(void *h, int n, const double *w_d, const double *x_d, double *out_d,
+                          double *dx_d, double *phi_d);
 /* host-pointer form; returns E, fills vir[6], dx[3n], phi[n] */
 double sfac_eval(void *h, int n, const double *w, const double *x, double vir[6],
                  double *dx, double *phi);
