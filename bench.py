@@ -66,8 +66,8 @@ def kernel_bytes(kernel, n, K, p):
     G = K0 * K1 * K2
     H = K0 * K1 * (K2 // 2 + 1)
     table = {
-        "prep": n * 4 * 8 + n * 8,             # x, w in; column, rank out
-        "scatter": n * (4 * 8 + 8) + n * (3 * p + 16),  # x, w, col, rank in; records out
+        "bin": n * 4 * 8 + n * (3 * p + 16),   # x, w in; records out (plane buckets)
+        "sort": 2 * n * (3 * p + 16),          # records in, records out in column order
         "spread": n * (3 * p + 16) + G * p,    # column-sorted records in, grid out once
         "fft_z_r2c": G * p + 2 * H * p,
         "fft_y_fwd": 4 * H * p,
@@ -494,9 +494,8 @@ def cpu_baseline(sy, budget_s=15.0):
 
 # kernel symbol prefix -> stage name of the batched FP32 evaluation
 STAGE_OF_KERNEL = {
-    "spme::prep_kernel": "prep",
-    "spme::scan_kernel": "scan",
-    "spme::scatter_kernel": "scatter",
+    "spme::bin_kernel": "bin",
+    "spme::sort_kernel": "sort",
     "spme::spread_tile_kernel": "spread",
     "spme::fft_yz_fwd_kernel": "fft_yz_fwd",
     "spme::fft_z_r2c_kernel": "fft_z_r2c",

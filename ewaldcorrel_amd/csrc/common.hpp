@@ -134,13 +134,15 @@ struct FrameDesc {
     const double *w, *x;
     double *dx, *phi;
     double *out;  // E, virial[6]
-    // column sort + spreading workspace
-    int *colcount;   // K0*K1 counters, zero between evaluations
-    int *colstart;   // K0*K1 + 1 exclusive prefix
-    int *a_col, *a_rank;
-    void *recs;      // AtomRec<T>[n] in column order
-    double *wmax;    // max |w| of each prep workgroup (fixed point), reduced by the scan
-    double *fxscale;              // fixed-point scale, written by the scan
+    // sort + spreading workspace
+    int *planecount;  // K0 plane counters and the overflow count, zero between evaluations
+    int *colstart;    // K0*K1 + 1 exclusive prefix
+    void *bucket;     // AtomRec<T>[K0][bcap]: each plane's records in bin order
+    int bcap;         // records a plane's bucket holds
+    int *ovf;         // indices of the atoms beyond their plane's bucket
+    void *recs;       // AtomRec<T>[n] in column order
+    double *wmax;     // max |w| of each bin workgroup (fixed point), reduced by the sort
+    double *fxscale;  // fixed-point scale, written by the sort
     double mmax3;
     int fixed;
     int fx32;        // FP32 spreading: int32 contributions (|q| < 2^30) into int64 sums

@@ -43,7 +43,7 @@ void hip_check(hipError_t e, const char *what) {
 
 const char *stage_name(Stage s) {
     static const char *names[ST_COUNT] = {
-        "prep",      "scan",      "scatter",   "spread",    "fft_z_r2c",
+        "bin",       "sort",      "spread",    "fft_z_r2c",
         "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
         "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
         "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile"};
@@ -144,9 +144,8 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
               hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
     const dim3 g(blocks), bl(kBlock);
     switch (s) {
-        case ST_PREP: prep_kernel<<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
-        case ST_SCAN: scan_kernel<<<g, 1024, 0, st>>>(L.frames, L.offs, L.nf); break;
-        case ST_SCATTER: scatter_kernel<T><<<g, bl, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_BIN: bin_kernel<T><<<g, kBinThreads, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_SORT: sort_kernel<T><<<g, kSortThreads, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_SPREAD: {
             const dim3 sb(kSpreadBlock);
             auto go = [&](auto fixed_c, auto zch_c) {
@@ -235,18 +234,18 @@ int finishes(Stage prev, Stage s) {
     return 0;
 }
 
-const Stage kSfacLines[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE};
-const Stage kSfacSlab[] = {ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_YZ_FWD, ST_XF_SCALE};
+const Stage kSfacLines[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE};
+const Stage kSfacSlab[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_YZ_FWD, ST_XF_SCALE};
 const Stage kDe1Lines[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
 const Stage kDe1Slab[] = {ST_INFL, ST_XI, ST_YZ_INV};
 const Stage kEnStages[] = {ST_INFL_E, ST_FINISH};
-const Stage kEvalLines[] = {ST_PREP, ST_SCAN,   ST_SCATTER, ST_SPREAD, ST_ZR2C,
+const Stage kEvalLines[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
                             ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_SORTED};
-const Stage kEvalSlab[] = {ST_PREP,   ST_SCAN,   ST_SCATTER, ST_SPREAD,
+const Stage kEvalSlab[] = {ST_BIN, ST_SORT, ST_SPREAD,
                            ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_SORTED};
-const Stage kEvalLinesTile[] = {ST_PREP, ST_SCAN,   ST_SCATTER, ST_SPREAD, ST_ZR2C,
+const Stage kEvalLinesTile[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
                                 ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_TILE};
-const Stage kEvalSlabTile[] = {ST_PREP,   ST_SCAN,   ST_SCATTER, ST_SPREAD,
+const Stage kEvalSlabTile[] = {ST_BIN, ST_SORT, ST_SPREAD,
                                ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_TILE};
 
 template <size_t NS> StageSeq seq_of(const Stage (&a)[NS]) { return StageSeq{a, (int)NS}; }
@@ -543,13 +542,11 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     HIPC(hipGetDevice(&dev_));
     memset(&desc_uploaded_, 0xff, sizeof(FrameDesc));
     alloc_((void **)&D.partials, (size_t)nred_max * 8 * sizeof(double));
-    // the scan reads the counters in zero-padded tiles of 1024 x 16
-    const size_t ncount = ((size_t)nrows + 16383) / 16384 * 16384;
-    alloc_((void **)&D.colcount, ncount * sizeof(int));
+    alloc_((void **)&D.planecount, ((size_t)K[0] + 1) * sizeof(int));
     alloc_((void **)&D.colstart, ((size_t)nrows + 1) * sizeof(int));
     alloc_((void **)&D.fxscale, 64);
     alloc_((void **)&result_, 8 * sizeof(double));
-    HIPC(hipMemsetAsync(D.colcount, 0, ncount * sizeof(int), stream_));
+    HIPC(hipMemsetAsync(D.planecount, 0, ((size_t)K[0] + 1) * sizeof(int), stream_));
     HIPC(hipMemsetAsync(grid_, 0, grid_reals() * tsz, stream_));
     HIPC(hipMemsetAsync(result_, 0, 8 * sizeof(double), stream_));
     D.out = result_;
@@ -736,25 +733,32 @@ void Engine::ensure_atoms(int n) {
     }
 }
 
-// The per-atom sort workspace of n atoms in one block: column ids and ranks,
-// the column-ordered records and one |w| maximum per prep workgroup, each
-// part 256-byte aligned.  Nothing in it needs to start at zero.
+// The per-atom sort workspace of n atoms in one block: the plane buckets
+// (K0 of bucket_cap(n) records: 1.5 times a uniform plane's share plus 64, so
+// that an even distribution never overflows), the overflow list (n indices),
+// the column-ordered records and one |w| maximum per bin workgroup, each part
+// 256-byte aligned.  Nothing in it needs to start at zero.
 namespace {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
+int Engine::bucket_cap(size_t n) const {
+    return (int)((3 * n + 2 * (size_t)K[0] - 1) / (2 * (size_t)K[0]) + 64);
+}
+
 size_t Engine::workspace_bytes(size_t n) const {
     const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
-    return 2 * align256(n * sizeof(int)) + align256(n * rec) +
-           align256((n + kBlock - 1) / kBlock * sizeof(double));
+    return align256((size_t)K[0] * bucket_cap(n) * rec) + align256(n * sizeof(int)) +
+           align256(n * rec) + align256((n + kBinAtoms - 1) / kBinAtoms * sizeof(double));
 }
 
 void Engine::bind_workspace(FrameDesc &D, void *base, size_t n) const {
     const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
     char *p = static_cast<char *>(base);
-    D.a_col = reinterpret_cast<int *>(p);
-    p += align256(n * sizeof(int));
-    D.a_rank = reinterpret_cast<int *>(p);
+    D.bcap = bucket_cap(n);
+    D.bucket = p;
+    p += align256((size_t)K[0] * D.bcap * rec);
+    D.ovf = reinterpret_cast<int *>(p);
     p += align256(n * sizeof(int));
     D.recs = p;
     p += align256(n * rec);
@@ -777,9 +781,8 @@ int Engine::stage_blocks_n(Stage s, int natoms) const {
     const FrameDesc &D = desc_;
     const int nrows = K[0] * K[1];
     switch (s) {
-        case ST_PREP:
-        case ST_SCATTER: return (natoms + kBlock - 1) / kBlock;
-        case ST_SCAN: return 1;
+        case ST_BIN: return (natoms + kBinAtoms - 1) / kBinAtoms;
+        case ST_SORT: return K[0];
         case ST_SPREAD: return D.ntx * D.nty * D.ntz;
         case ST_ZR2C:
         case ST_ZC2R: return (nrows + 2 * D.zC - 1) / (2 * D.zC);
@@ -809,6 +812,8 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
     // frame with a plain plan in the generic kernel needs its second line set
     const int xgen = std::max(xpath, x_path(D)) == kXGeneric;
     switch (s) {
+        case ST_SORT:  // the staged plane
+            return fp32 ? sort_lds_bytes<float>(D.bcap) : sort_lds_bytes<double>(D.bcap);
         case ST_SPREAD:  // chunked tiles: the compacted atom list after the cells
             return (size_t)D.tx * D.ty * (D.ntz > 1 ? D.tz : K[2] + order - 1) * 8 +
                    (D.ntz > 1 ? kSpreadBlock * sizeof(int) : 0);
@@ -1005,7 +1010,11 @@ void Engine::evaluate_noalloc(int n, const double *w_d, const double *x_d, doubl
     Stage prev = ST_COUNT;
     for (int i = 0; i < seq.n; ++i) {
         const Stage st = seq.s[i];
-        launch(st, L, stage_blocks_n(st, n), stage_lds(st), stream_, finishes(prev, st));
+        // the sort's LDS follows the bucket capacity of this call's workspace
+        const size_t lds = st != ST_SORT ? stage_lds(st)
+                           : fp32        ? sort_lds_bytes<float>(desc_na_.bcap)
+                                         : sort_lds_bytes<double>(desc_na_.bcap);
+        launch(st, L, stage_blocks_n(st, n), lds, stream_, finishes(prev, st));
         prev = st;
     }
     state = 1;

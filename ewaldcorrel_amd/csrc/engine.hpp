@@ -25,12 +25,12 @@ enum EngineFlags {
     kFlagInfl32 = 4,      // FP32 engine: closed-form influence and thread sums in FP32
 };
 
-// Kernel stages.  An evaluation runs PREP..SPREAD, YZ_FWD, XFUSED, YZ_INV,
-// GATHER_SORTED (or ZR2C, YF in place of YZ_FWD and YI, ZC2R in place of
-// YZ_INV when the y-z slab does not fit in LDS); the split entry points use
-// XF_SCALE / INFL / INFL_E / XI.
+// Kernel stages.  An evaluation runs BIN, SORT, SPREAD, YZ_FWD, XFUSED, YZ_INV,
+// GATHER_TILE or GATHER_SORTED (or ZR2C, YF in place of YZ_FWD and YI, ZC2R in
+// place of YZ_INV when the y-z slab does not fit in LDS); the split entry
+// points use XF_SCALE / INFL / INFL_E / XI.
 enum Stage {
-    ST_PREP, ST_SCAN, ST_SCATTER, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
+    ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
     ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
     ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_COUNT
 };
@@ -207,6 +207,7 @@ public:
     int stage_blocks_n(Stage s, int natoms) const;  // for natoms atoms
     // per-atom sort workspace: bytes for n atoms, and its parts in D at base
     size_t workspace_bytes(size_t n) const;
+    int bucket_cap(size_t n) const;  // records per plane bucket for n atoms
     void bind_workspace(FrameDesc &D, void *base, size_t n) const;
     // xpath: the x path of the launch (kXBlock, or kXGeneric for a batch with
     // a generic frame); the frame's own path when that is more general

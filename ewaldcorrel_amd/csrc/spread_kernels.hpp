@@ -1,19 +1,24 @@
-// Charge spreading: column sort + LDS-tiled owner-computes scatter.
+// Charge spreading: two-level counting sort + LDS-tiled owner-computes scatter.
 //
 // The reference spreads in gather form, one grid point at a time, with
 // order^3 multimap lookups per point (sfac.cpp:113-124,141-168).  Here:
-//   prep     per atom: cell c = (int)(s K), column (cx, cy), rank within the
-//            column (one counter atomic), block max |w|
-//   scan     per frame: exclusive prefix of the column counts, counters
-//            cleared for the next call, fixed-point scale fixed
-//   scatter  per atom: one record (fraction, w, cell, original index) in
-//            column order
+//   bin      per atom: cell c = (int)(s K) and its record (fraction, w, cell,
+//            original index) into the bucket of its x plane cx; a workgroup
+//            counts its atoms per plane in LDS and reserves each plane's slots
+//            with one global atomic (not one per atom); atoms beyond a
+//            bucket's capacity go to an overflow list (by index); block max |w|
+//   sort     per (frame, x plane): counting sort of the plane's records by
+//            column cy in LDS; writes the column starts and the records in
+//            column order (frame-wide: planes in x order, columns in y
+//            order); plane 0 also fixes the fixed-point scale
 //   spread   per x-y tile of full-z columns: every atom of the
 //            (tile + order - 1) neighbourhood adds the part of its stencil
 //            that falls in the tile to an LDS accumulator (int64 fixed
 //            point or FP64 LDS atomics); the tile is then written once with
 //            coalesced stores.  No global atomics, no separate zeroing or
 //            conversion pass, and every grid point is written exactly once.
+//            The first tile of each frame clears the plane counters for the
+//            next call.
 // The stencil convention is the reference's: an atom in cell c with
 // fraction f adds w M(f0+i) M(f1+j) M(f2+k) to ((c + n/2 - i) mod K, ...).
 #pragma once
@@ -75,17 +80,6 @@ __device__ __forceinline__ void atom_cell(const FrameDesc &F, int a, int c[3], d
     if (c[2] >= F.K2) c[2] -= F.K2;
 }
 
-__device__ __forceinline__ double block_max(double v, double *red) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    double m = 0.0;
-    if (threadIdx.x == 0)
-        for (int q = 0; q < kBlock / 64; ++q) m = fmax(m, red[q]);
-    return m;
-}
-
 // FP64 values: scale = 2^(62 - e) with n*max|w| < 2^e; an atom adds at most
 // |w| to any one grid point in total (the spline weights of each axis sum to
 // one, also when a grid shorter than the stencil wraps it onto itself), so no
@@ -101,131 +95,12 @@ __device__ __forceinline__ double fixed_scale(int n, double wmax, double mmax3, 
     return ldexp(1.0, (fx32 ? 30 : 62) - e);
 }
 
-// ---- prep: one thread per atom -------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) prep_kernel(SPME_FRAME_ARGS) {
-    __shared__ double red[kBlock / 64];
-    int b;
-    const FrameDesc &F = frames[find_frame(offs, nf, b)];
-    const int a = b * kBlock + threadIdx.x;
-    double wa = 0.0;
-    if (a < F.n) {
-        int c[3];
-        double f[3];
-        atom_cell(F, a, c, f);
-        const int col = c[0] * F.K1 + c[1];
-        F.a_col[a] = col;
-        F.a_rank[a] = atomicAdd(F.colcount + col, 1);
-        wa = fabs(F.w[a]);
-    }
-    // one partial maximum per workgroup (a single atomic address for the
-    // whole frame serialises hundreds of workgroups at memory)
-    if (F.fixed) {
-        const double m = block_max(wa, red);
-        if (threadIdx.x == 0) F.wmax[b] = m;
-    }
-}
-
-// ---- scan: one workgroup per frame ----------------------------------------------------
-// Tiles of 1024 x 16 columns: each thread loads its 16 consecutive counts at
-// once (all loads in flight), the thread sums are scanned within each
-// wavefront by shuffles, the 16 wavefront totals through LDS (two barriers
-// per tile), and the tile's offsets continue from the running total of
-// earlier tiles.
-__global__ void __launch_bounds__(1024) scan_kernel(SPME_FRAME_ARGS) {
-    constexpr int E = 16;
-    __shared__ int wtot[1024 / 64];
-    const FrameDesc &F = frames[nf <= 1 ? 0 : blockIdx.x];
-    const int ncol = F.K0 * F.K1;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // the prep workgroups' partial maxima of |w|, loaded before the scan so
-    // that their latency overlaps it
-    double m = 0.0;
-    if (F.fixed) {
-        const int nb = (F.n + kBlock - 1) / kBlock;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, F.wmax[i]);
-    }
-    int carry = 0;
-    for (int base = 0; base < ncol; base += 1024 * E) {
-        const int i0 = base + (int)threadIdx.x * E;
-        // colcount is zero-padded to a multiple of 1024*E: four unconditional
-        // 16-byte loads per thread, all in flight together
-        int v[E];
-        const int4 *src = reinterpret_cast<const int4 *>(F.colcount + i0);
-#pragma unroll
-        for (int e4 = 0; e4 < E / 4; ++e4) {
-            const int4 q = src[e4];
-            v[4 * e4 + 0] = q.x;
-            v[4 * e4 + 1] = q.y;
-            v[4 * e4 + 2] = q.z;
-            v[4 * e4 + 3] = q.w;
-        }
-        int s = 0;
-#pragma unroll
-        for (int e = 0; e < E; ++e) s += v[e];
-        int incl = s;  // inclusive scan within the wavefront
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
-        if (lane == 63) wtot[w] = incl;
-        __syncthreads();
-        int woff = 0, tot = 0;
-#pragma unroll
-        for (int q = 0; q < 1024 / 64; ++q) {
-            const int c = wtot[q];
-            woff += q < w ? c : 0;
-            tot += c;
-        }
-        int run = carry + woff + incl - s;
-        if (i0 + E <= ncol) {  // four 16-byte stores (i0 is a multiple of 16)
-            int o[E];
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                o[e] = run;
-                run += v[e];
-            }
-            int4 *cs = reinterpret_cast<int4 *>(F.colstart + i0);
-#pragma unroll
-            for (int e4 = 0; e4 < E / 4; ++e4)
-                cs[e4] = make_int4(o[4 * e4], o[4 * e4 + 1], o[4 * e4 + 2], o[4 * e4 + 3]);
-        } else {
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                if (i0 + e < ncol) F.colstart[i0 + e] = run;
-                run += v[e];
-            }
-        }
-        int4 *dst = reinterpret_cast<int4 *>(F.colcount + i0);
-#pragma unroll
-        for (int e4 = 0; e4 < E / 4; ++e4) dst[e4] = make_int4(0, 0, 0, 0);
-        carry += tot;
-        __syncthreads();  // wtot is rewritten by the next tile
-    }
-    if (threadIdx.x == 0) F.colstart[ncol] = carry;
-    if (F.fixed) {  // max |w| over the prep workgroups' partial maxima
-        __shared__ double wred[1024 / 64];
-        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-        if (lane == 0) wred[w] = m;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int q = 1; q < 1024 / 64; ++q) m = fmax(m, wred[q]);
-            *F.fxscale = fixed_scale(F.n, m, F.mmax3, F.fx32);
-        }
-    }
-}
-
-// ---- scatter: one thread per atom -------------------------------------------------------
+// the record of atom a (scale_wrap, cell and fraction as atom_cell)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) scatter_kernel(SPME_FRAME_ARGS) {
-    int b;
-    const FrameDesc &F = frames[find_frame(offs, nf, b)];
-    const int a = b * kBlock + threadIdx.x;
-    if (a >= F.n) return;
+__device__ __forceinline__ AtomRec<T> make_rec(const FrameDesc &F, int a, int &cx) {
     int c[3];
     double f[3];
     atom_cell(F, a, c, f);
-    const int pos = F.colstart[F.a_col[a]] + F.a_rank[a];
     AtomRec<T> r;
     r.f[0] = T(f[0]);
     r.f[1] = T(f[1]);
@@ -233,7 +108,204 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(SPME_FRAME_ARGS) {
     r.w = F.w[a];
     r.cell = (c[0] << 20) | (c[1] << 10) | c[2];
     r.perm = a;
-    ((AtomRec<T> *)F.recs)[pos] = r;
+    cx = c[0];
+    return r;
+}
+
+// ---- bin: atoms into x-plane buckets ------------------------------------------------
+// kBinAtoms atoms per workgroup (kBinAtoms / kBinThreads per thread, all loads
+// in flight together).  F.planecount[0..K0) counts each plane's atoms over the
+// whole frame (the bucket holds the first F.bcap of them), F.planecount[K0] the
+// overflow list's length; both are zero when the launch starts.
+constexpr int kBinThreads = 1024;
+constexpr int kBinAtoms = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
+    constexpr int A = kBinAtoms / kBinThreads;
+    __shared__ int pl[kMaxK];  // per-plane count, then this workgroup's base in the plane
+    __shared__ double red[kBinThreads / 64];
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const int K0 = F.K0;
+    for (int p = threadIdx.x; p < K0; p += blockDim.x) pl[p] = 0;
+    __syncthreads();
+    AtomRec<T> r[A];
+    int cx[A], lr[A];
+    double wm = 0.0;
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+        const int a = b * kBinAtoms + q * kBinThreads + (int)threadIdx.x;
+        cx[q] = -1;
+        if (a < F.n) {
+            r[q] = make_rec<T>(F, a, cx[q]);
+            wm = fmax(wm, fabs(r[q].w));
+            lr[q] = atomicAdd(&pl[cx[q]], 1);
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < K0; p += blockDim.x) {
+        const int h = pl[p];
+        if (h) pl[p] = atomicAdd(F.planecount + p, h);
+    }
+    __syncthreads();
+    AtomRec<T> *bucket = (AtomRec<T> *)F.bucket;
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+        if (cx[q] < 0) continue;
+        const int slot = pl[cx[q]] + lr[q];
+        if (slot < F.bcap) {
+            bucket[(long)cx[q] * F.bcap + slot] = r[q];
+        } else {  // a crowded plane: the atom goes to the overflow list
+            const int o = atomicAdd(F.planecount + K0, 1);
+            F.ovf[o] = r[q].perm;
+        }
+    }
+    if (F.fixed) {  // one partial maximum per workgroup
+        for (int off = 32; off > 0; off >>= 1) wm = fmax(wm, __shfl_xor(wm, off));
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) red[wv] = wm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int k = 0; k < kBinThreads / 64; ++k) m = fmax(m, red[k]);
+            F.wmax[b] = m;
+        }
+    }
+}
+
+// ---- sort: one workgroup per (frame, x plane) ------------------------------------
+// Counting sort of the plane's bucket (and of its atoms on the overflow list)
+// by column: counts per column in LDS, an exclusive scan, the column starts
+// (offset by the atoms of all earlier planes), then every record to its place
+// through per-column LDS cursors.  The order within a column is the order of
+// the cursor atomics; no result depends on it (fixed-point sums are exact
+// integers and every atom's gather is its own).
+constexpr int kSortThreads = 512;
+constexpr int kSortRegs = 4;  // records per thread held in registers (staged sort)
+
+// LDS of the staged sort: a plane's bucket in column order (0: the planes are
+// sorted in place, directly from the bucket to the output)
+template <typename T> __host__ __device__ inline size_t sort_lds_bytes(int bcap) {
+    const size_t b = (size_t)bcap * sizeof(AtomRec<T>);
+    return bcap <= kSortThreads * kSortRegs && b <= 64 * 1024 ? b : 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int cur[kMaxK];  // column counts, then their starts, then cursors
+    __shared__ int wtot[kSortThreads / 64];
+    __shared__ double wred[kSortThreads / 64];
+    int cx;
+    const FrameDesc &F = frames[find_frame(offs, nf, cx)];
+    const int K0 = F.K0, K1 = F.K1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int NW = kSortThreads / 64;
+    const int cnt = F.planecount[cx];
+    const int nb = min(cnt, F.bcap);
+    const int novf = F.planecount[K0];
+    const AtomRec<T> *bk = (const AtomRec<T> *)F.bucket + (long)cx * F.bcap;
+    // staged: the whole plane from its bucket into registers (one read), its
+    // records placed in column order in LDS, then written out contiguously
+    const bool staged = novf == 0 && sort_lds_bytes<T>(F.bcap) > 0;
+    AtomRec<T> r[kSortRegs];
+    if (staged) {
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q) {
+            const int i = q * kSortThreads + (int)threadIdx.x;
+            if (i < nb) r[q] = bk[i];
+        }
+    }
+    // atoms in the planes before this one
+    int part = 0;
+    for (int p = threadIdx.x; p < cx; p += blockDim.x) part += F.planecount[p];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0) wtot[wv] = part;
+    for (int y = threadIdx.x; y < K1; y += blockDim.x) cur[y] = 0;
+    __syncthreads();
+    int pre = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) pre += wtot[k];
+    if (staged) {
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q)
+            if (q * kSortThreads + (int)threadIdx.x < nb) atomicAdd(&cur[(r[q].cell >> 10) & 1023], 1);
+    } else {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x)
+            atomicAdd(&cur[(bk[i].cell >> 10) & 1023], 1);
+        for (int i = threadIdx.x; i < novf; i += blockDim.x) {
+            int c;
+            const AtomRec<T> o = make_rec<T>(F, F.ovf[i], c);
+            if (c == cx) atomicAdd(&cur[(o.cell >> 10) & 1023], 1);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of cur[0..K1): two consecutive columns per thread
+    {
+        const int y0 = 2 * (int)threadIdx.x;
+        const int v0 = y0 < K1 ? cur[y0] : 0, v1 = y0 + 1 < K1 ? cur[y0 + 1] : 0;
+        const int sm = v0 + v1;
+        int incl = sm;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        __syncthreads();  // every thread has read its pair of counts and wtot
+        if (lane == 63) wtot[wv] = incl;
+        __syncthreads();
+        int woff = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) woff += k < wv ? wtot[k] : 0;
+        const int e0 = woff + incl - sm;  // start of column y0 within the plane
+        if (y0 < K1) {
+            cur[y0] = e0;
+            F.colstart[cx * K1 + y0] = pre + e0;
+        }
+        if (y0 + 1 < K1) {
+            cur[y0 + 1] = e0 + v0;
+            F.colstart[cx * K1 + y0 + 1] = pre + e0 + v0;
+        }
+        if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + cnt;
+    }
+    __syncthreads();
+    AtomRec<T> *recs = (AtomRec<T> *)F.recs + pre;
+    if (staged) {
+        AtomRec<T> *sbuf = reinterpret_cast<AtomRec<T> *>(smem_raw);
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q)
+            if (q * kSortThreads + (int)threadIdx.x < nb)
+                sbuf[atomicAdd(&cur[(r[q].cell >> 10) & 1023], 1)] = r[q];
+        __syncthreads();
+        // contiguous write-out, 16 bytes per lane
+        const uint4 *src = reinterpret_cast<const uint4 *>(sbuf);
+        uint4 *dst = reinterpret_cast<uint4 *>(recs);
+        constexpr int W = sizeof(AtomRec<T>) / sizeof(uint4);
+        for (int i = threadIdx.x; i < nb * W; i += blockDim.x) dst[i] = src[i];
+    } else {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            const AtomRec<T> o = bk[i];
+            recs[atomicAdd(&cur[(o.cell >> 10) & 1023], 1)] = o;
+        }
+        for (int i = threadIdx.x; i < novf; i += blockDim.x) {
+            int c;
+            const AtomRec<T> o = make_rec<T>(F, F.ovf[i], c);
+            if (c == cx) recs[atomicAdd(&cur[(o.cell >> 10) & 1023], 1)] = o;
+        }
+    }
+    if (cx == 0 && F.fixed) {  // the fixed-point scale, from the bin partial maxima
+        const int nbin = (F.n + kBinAtoms - 1) / kBinAtoms;
+        double m = 0.0;
+        for (int i = threadIdx.x; i < nbin; i += blockDim.x) m = fmax(m, F.wmax[i]);
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if (lane == 0) wred[wv] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < NW; ++k) m = fmax(m, wred[k]);
+            *F.fxscale = fixed_scale(F.n, m, F.mmax3, F.fx32);
+        }
+    }
 }
 
 // ---- spread: one workgroup per x-y tile -------------------------------------------------------
@@ -280,6 +352,8 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
 
     typedef typename std::conditional<FIXED, unsigned long long, double>::type Acc;
     Acc *acc = reinterpret_cast<Acc *>(smem_raw);
+    if (b == 0)  // the sort has consumed the plane counters: cleared for the next call
+        for (int p = threadIdx.x; p <= K0; p += blockDim.x) F.planecount[p] = 0;
     {
         uint4 *z4 = reinterpret_cast<uint4 *>(smem_raw);
         for (int i = threadIdx.x; i < cells / 2; i += blockDim.x) z4[i] = make_uint4(0, 0, 0, 0);
