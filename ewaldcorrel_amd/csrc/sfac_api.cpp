@@ -71,9 +71,33 @@ SFac::SFac(double L[6], int32_t K_[3], int order_, int flags)
     for (int d = 0; d < 3; ++d) iB.push_back(engine->iB[d]);
 }
 
+SFac::SFac(double L[6], int32_t K_[3], int order_) : SFac(L, K_, order_, 0) {}
+
 SFac::~SFac() {
     delete engine;
     free(A);
+}
+
+void SFac::accum_spline(int i1, int j1, int k1, const std::multimap<int, Vec4> &srt) {
+    if (!Q) Q = engine->get_S_host();
+    const long NN = ((long)i1 * K[1] + j1) * ldim * 2 + k1;
+    const int i0 = i1 + K[0] - order / 2, j0 = j1 + K[1] - order / 2, k0 = k1 + K[2] - order / 2;
+    double ans = 0.0;
+    for (int i = 0; i < order; ++i) {
+        const int xid = ((i0 + i) % K[0]) * K[1];
+        for (int j = 0; j < order; ++j) {
+            const int yid = (xid + (j0 + j) % K[1]) * K[2];
+            for (int k = 0; k < order; ++k) {
+                auto range = srt.equal_range(yid + (k0 + k) % K[2]);
+                for (auto it = range.first; it != range.second; ++it) {
+                    const Vec4 &u = it->second;
+                    ans += bspc.bspl_coef(u[0], i) * bspc.bspl_coef(u[1], j) *
+                           bspc.bspl_coef(u[2], k) * u.w;
+                }
+            }
+        }
+    }
+    Q[NN] += ans;
 }
 
 std::multimap<int, Vec4> SFac::sort(int n, const double *w, const double *x) const {

@@ -148,12 +148,21 @@ struct SFac {
     };
     spme::Engine *engine;
 
-    SFac(double L[6], int32_t K[3], int order, int flags = 0);
+    // the reference's constructor (sfac.hpp:185, same mangled symbol), and
+    // one with the precision flags of sfac_ctor_ex
+    SFac(double L[6], int32_t K[3], int order);
+    SFac(double L[6], int32_t K[3], int order, int flags);
     ~SFac();
     // Cell sort of the reference (sfac.cpp:92-111): cell id -> (frac, w).
     std::multimap<int, Vec4> sort(int n, const double *w, const double *x) const;
     // spread -> r2c -> /B on the device (sfac.cpp:113-139)
     void operator()(int n, const double *w, const double *x);
+    // The reference's gather-form spreading of one grid point (sfac.cpp:141-168):
+    // adds sum over the sorted atoms of w M(f0+i) M(f1+j) M(f2+k) from cells
+    // (i1 - n/2 + i, ...) mod K to Q at (i1, j1, k1).  It acts on the host
+    // mirror Q (the grid as last copied by get_S, taken now if there is none),
+    // not on the device grid that operator() spreads into.
+    void accum_spline(int i1, int j1, int k1, const std::multimap<int, Vec4> &srt);
 };
 
 extern "C" {

@@ -131,3 +131,49 @@ def test_tune_records_round_trip_without_device():
     assert "32x32x32" not in out
     ewald.tune_clear()
     assert ewald.tune_export() == ""
+
+
+def _m4(x):
+    """Closed-form cubic cardinal B-spline M_4 on [0, 4]."""
+    if x < 0 or x >= 4:
+        return 0.0
+    if x < 1:
+        return x ** 3 / 6
+    if x < 2:
+        return (-3 * x ** 3 + 12 * x ** 2 - 12 * x + 4) / 6
+    if x < 3:
+        return (3 * x ** 3 - 24 * x ** 2 + 60 * x - 44) / 6
+    return (4 - x) ** 3 / 6
+
+
+def test_cxx_caller_links_and_runs(tmp_path):
+    """A C++ program shaped like the reference's test.cpp compiles against
+    include/sfac.hpp with g++, links libsfac.so (including the reference's
+    SFac(double*, int*, int) constructor symbol) and runs without a GPU."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    exe = tmp_path / "caller"
+    src = os.path.join(ROOT, "tests", "cpp", "test_caller.cpp")
+    libdir = os.path.dirname(LIB)
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    src, "-o", str(exe), "-L", libdir, "-lsfac", f"-Wl,-rpath,{libdir}"],
+                   check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True,
+                         timeout=120).stdout.splitlines()
+    table = [ln.split() for ln in out if ln[:1].isdigit()]
+    assert len(table) == 40
+    for xs, c in table:
+        assert abs(float(c) - _m4(float(xs))) <= 6e-7, (xs, c)
+    ds = [ln.split() for ln in out if ln.startswith("d ")]
+    for _, j, b, db in ds:
+        x = 0.3 + int(j)
+        h = 1e-6
+        assert abs(float(b) - _m4(x)) <= 1e-14
+        assert abs(float(db) - (_m4(x + h) - _m4(x - h)) / (2 * h)) <= 1e-8
+    joined = "\n".join(out)
+    assert "ctor_odd_order NULL invalid order" in joined
+    assert "ctor_bad_K NULL invalid K" in joined
+    assert "cxx_ctor throws invalid order" in joined
