@@ -202,9 +202,12 @@ __device__ __forceinline__ int find_frame(const int *__restrict__ offs, int nf, 
 }
 
 // Fractional coordinates wrapped to [0,1): s = wrap(iL^T x) (sfac.hpp:56-62).
+// No contraction: every kernel (and the host) that computes an atom's cell
+// and fraction gets the same bits, whichever of them does it.
 __host__ __device__ inline void scale_wrap(const CellArgs &c, double x0, double x1,
                                            double x2, double &s0, double &s1,
                                            double &s2) {
+#pragma clang fp contract(off)
     s0 = x0 * c.iL00 + x1 * c.iL10 + x2 * c.iL20;
     s1 = x1 * c.iL11 + x2 * c.iL21;
     s2 = x2 * c.iL22;

@@ -1493,12 +1493,10 @@ void Engine::accum_cross(const Engine &b, double *acc_d) {
 
 void Engine::radial(const double *val_d, int nbins, double dk, double *sum_d, double *cnt_d) {
     if (nbins < 1 || !(dk > 0.0)) throw std::invalid_argument("radial: nbins >= 1 and dk > 0");
-    const size_t h = H();
-    const std::vector<double> key = {L6[0], L6[1], L6[2], L6[3], L6[4], L6[5], (double)nbins, dk};
+    const std::vector<double> key = {0.0, L6[0], L6[1], L6[2], L6[3], L6[4], L6[5],
+                                     (double)nbins, dk};
     if (key != radial_key_) {
-        // bin of every point, then a stable counting sort by bin
-        std::vector<int> bin(h);
-        std::vector<int> off((size_t)nbins + 1, 0);
+        std::vector<int> bin(H());
         for (int kx = 0; kx < K[0]; ++kx)
             for (int ky = 0; ky < K[1]; ++ky)
                 for (int kz = 0; kz < ldim; ++kz) {
@@ -1509,28 +1507,73 @@ void Engine::radial(const double *val_d, int nbins, double dk, double *sum_d, do
                     const double mz = nx * cell.iL20 + ny * cell.iL21 + kz * cell.iL22;
                     const double k = 2.0 * M_PI * sqrt(mx * mx + my * my + mz * mz);
                     const double bf = floor(k / dk);
-                    const int b = bf < (double)nbins ? (int)bf : -1;
-                    const size_t p = ((size_t)kx * K[1] + ky) * ldim + kz;
-                    bin[p] = b;
-                    if (b >= 0) off[(size_t)b + 1] += 1;
+                    bin[((size_t)kx * K[1] + ky) * ldim + kz] = bf < (double)nbins ? (int)bf : -1;
                 }
-        for (int b = 0; b < nbins; ++b) off[(size_t)b + 1] += off[b];
-        std::vector<int> perm(std::max<size_t>(off[nbins], 1));
-        std::vector<int> pos(off.begin(), off.end() - 1);
-        for (size_t p = 0; p < h; ++p)
-            if (bin[p] >= 0) perm[pos[bin[p]]++] = (int)p;
-        if (!radial_perm_) alloc_((void **)&radial_perm_, h * sizeof(int));
-        if ((size_t)nbins + 1 > radial_off_cap_) {
-            alloc_((void **)&radial_off_, ((size_t)nbins + 1) * sizeof(int));
-            radial_off_cap_ = (size_t)nbins + 1;
-        }
-        HIPC(hipMemcpyAsync(radial_perm_, perm.data(), perm.size() * sizeof(int),
-                            hipMemcpyHostToDevice, stream_));
-        HIPC(hipMemcpyAsync(radial_off_, off.data(), off.size() * sizeof(int),
-                            hipMemcpyHostToDevice, stream_));
-        HIPC(hipStreamSynchronize(stream_));
-        radial_key_ = key;
+        radial_bins_(key, bin, nbins);
     }
+    radial_sum_(val_d, nbins, sum_d, cnt_d);
+}
+
+void Engine::radial_plot(const double *val_d, double dk, double clip, int nbins, double *sum_d,
+                         double *cnt_d) {
+#pragma clang fp contract(off)
+    if (nbins < 1 || !(dk > 0.0) || !(clip >= 0.0))
+        throw std::invalid_argument("radial_plot: nbins >= 1, dk > 0 and clip >= 0");
+    const std::vector<double> key = {1.0, L6[0], L6[1], L6[2], (double)nbins, dk, clip};
+    if (key != radial_key_) {
+        // plot.py's arithmetic: k_d = i * pi * 2 / L_d, |k| = sqrt(kx^2 + ky^2 + kz^2)
+        auto kvec = [](int n, double L) {
+            std::vector<double> v(n);
+            for (int i = 0; i < n; ++i) {
+                const int j = i >= n / 2 ? i - n : i;
+                v[i] = (double)j * M_PI * 2.0 / L;
+            }
+            return v;
+        };
+        const std::vector<double> kx = kvec(K[0], L6[0]), ky = kvec(K[1], L6[1]),
+                                  kz = kvec(ldim, L6[2]);
+        std::vector<int> bin(H());
+        for (int x = 0; x < K[0]; ++x)
+            for (int y = 0; y < K[1]; ++y)
+                for (int z = 0; z < ldim; ++z) {
+                    const double s2 = kx[x] * kx[x] + ky[y] * ky[y] + kz[z] * kz[z];
+                    double d = sqrt(s2) / dk;
+                    if (d > clip) d = clip;
+                    const int b = (int)d;
+                    bin[((size_t)x * K[1] + y) * ldim + z] = b < nbins ? b : -1;
+                }
+        radial_bins_(key, bin, nbins);
+    }
+    radial_sum_(val_d, nbins, sum_d, cnt_d);
+}
+
+// points grouped by bin (a stable counting sort: increasing h within a bin)
+void Engine::radial_bins_(const std::vector<double> &key, const std::vector<int> &bin,
+                          int nbins) {
+    const size_t h = H();
+    std::vector<int> off((size_t)nbins + 1, 0);
+    for (size_t p = 0; p < h; ++p)
+        if (bin[p] >= 0) off[(size_t)bin[p] + 1] += 1;
+    for (int b = 0; b < nbins; ++b) off[(size_t)b + 1] += off[b];
+    std::vector<int> perm(std::max<size_t>(off[nbins], 1));
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (size_t p = 0; p < h; ++p)
+        if (bin[p] >= 0) perm[pos[bin[p]]++] = (int)p;
+    if (!radial_perm_) alloc_((void **)&radial_perm_, h * sizeof(int));
+    if ((size_t)nbins + 1 > radial_off_cap_) {
+        free_(radial_off_);
+        alloc_((void **)&radial_off_, ((size_t)nbins + 1) * sizeof(int));
+        radial_off_cap_ = (size_t)nbins + 1;
+    }
+    HIPC(hipMemcpyAsync(radial_perm_, perm.data(), perm.size() * sizeof(int),
+                        hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(radial_off_, off.data(), off.size() * sizeof(int),
+                        hipMemcpyHostToDevice, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    radial_key_ = key;
+}
+
+void Engine::radial_sum_(const double *val_d, int nbins, double *sum_d, double *cnt_d) {
     radial_kernel<<<(nbins + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(
         val_d, radial_perm_, radial_off_, nbins, sum_d, cnt_d);
     HIPC(hipGetLastError());

@@ -165,6 +165,12 @@ public:
     // sum_d[b] += sum of val_d over the points of |k| bin b, cnt_d[b] += count;
     // k = 2 pi m, m from the signed principal indices, b = floor(|k| / dk)
     void radial(const double *val_d, int nbins, double dk, double *sum_d, double *cnt_d);
+    // the same with plot.py's bins (plot.py:9-32): per axis of the stored
+    // K0 x K1 x ldim array, index i -> i - n for i >= n/2 (n the axis length,
+    // ldim for z), k_d = 2 pi i / L_d with the box's axis lengths, and bin
+    // b = (int) min(|k| / dk, clip); points with b >= nbins are left out
+    void radial_plot(const double *val_d, double dk, double clip, int nbins, double *sum_d,
+                     double *cnt_d);
 
     // real-space Ewald sum over the images within rc plus the self and
     // background terms, in the current box: out[10] = E_pair, vir(6), E_self,
@@ -275,6 +281,8 @@ private:
     std::vector<double> host_grid_;
     // |k| bins of the half-complex points for radial(): key, points by bin, offsets
     std::vector<double> radial_key_;
+    void radial_bins_(const std::vector<double> &key, const std::vector<int> &bin, int nbins);
+    void radial_sum_(const double *val_d, int nbins, double *sum_d, double *cnt_d);
     double *real_out_ = nullptr;    // real-space results of the host-pointer form
     double *real_parts_ = nullptr;  // real-space workgroup partials
     size_t real_parts_cap_ = 0;

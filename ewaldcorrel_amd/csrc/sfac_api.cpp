@@ -253,6 +253,12 @@ int sfac_accum_cross(void *ha, void *hb, double *acc_d) {
     return guard([&] { as_sfac(ha)->engine->accum_cross(*as_sfac(hb)->engine, acc_d); });
 }
 
+int sfac_radial_plot(void *h, const double *val_d, double dk, double clip, int nbins,
+                     double *sum_d, double *cnt_d) {
+    return guard(
+        [&] { as_sfac(h)->engine->radial_plot(val_d, dk, clip, nbins, sum_d, cnt_d); });
+}
+
 int sfac_radial(void *h, const double *val_d, int nbins, double dk, double *sum_d,
                 double *cnt_d) {
     return guard([&] { as_sfac(h)->engine->radial(val_d, nbins, dk, sum_d, cnt_d); });

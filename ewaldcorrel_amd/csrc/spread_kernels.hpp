@@ -66,6 +66,7 @@ __device__ __forceinline__ T spline_one(const SplineArgs<T> &sp, int n, T x, int
 // cell and fraction of one atom (sfac.cpp:97-104); a coordinate wrapped to
 // exactly 1.0 is the periodic image in cell 0.
 __device__ __forceinline__ void atom_cell(const FrameDesc &F, int a, int c[3], double f[3]) {
+#pragma clang fp contract(off)
     double s0, s1, s2;
     scale_wrap(F.cell, F.x[3 * a], F.x[3 * a + 1], F.x[3 * a + 2], s0, s1, s2);
     const double u0 = s0 * F.K0, u1 = s1 * F.K1, u2 = s2 * F.K2;

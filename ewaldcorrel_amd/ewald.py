@@ -100,6 +100,8 @@ def describe_sfac():
         int_fn(lib.sfac_accum_S2, sfac_p, c_void_p)
         int_fn(lib.sfac_accum_cross, sfac_p, sfac_p, c_void_p)
         int_fn(lib.sfac_radial, sfac_p, c_void_p, c_int, c_double, c_void_p, c_void_p)
+        int_fn(lib.sfac_radial_plot, sfac_p, c_void_p, c_double, c_double, c_int, c_void_p,
+               c_void_p)
         int_fn(lib.sfac_real_space, sfac_p, c_int, nparr, nparr, c_double, c_double,
                nparr_t(10), c_void_p, c_void_p)
         int_fn(lib.sfac_real_space_dev, sfac_p, c_int, c_void_p, c_void_p, c_double, c_double,
@@ -326,6 +328,36 @@ class Sfac:
         """|k| shell sums and counts of a per-point array (plot.py:19-32)."""
         _check(self._libsfac.sfac_radial(self._data, val_ptr, int(nbins), float(dk),
                                          sum_ptr, cnt_ptr), "radial")
+
+    def radial_plot(self, val_ptr, dk, clip, nbins, sum_ptr, cnt_ptr):
+        """|k| shell sums and counts with plot.py's k-vectors and clipping
+        (plot.py:9-32), on the device; see sfac_radial_plot."""
+        _check(self._libsfac.sfac_radial_plot(self._data, val_ptr, float(dk), float(clip),
+                                              int(nbins), sum_ptr, cnt_ptr), "radial_plot")
+
+    def plot_spectrum(self, val):
+        """plot.py's output for a (K0, K1, ldim) array of this handle's grid
+        (a device tensor or host array, e.g. the real part of a sum of
+        |S|^2): rows (k, mean, count) of its non-empty |k| shells, binned on
+        the device.  plot.py's box length is taken from the handle (L6[0])."""
+        import math
+        import torch
+        v = torch.as_tensor(val, dtype=torch.float64, device="cuda").reshape(-1).contiguous()
+        K0 = int(self.K[0])
+        L = float(self._L6[0])
+        kmax = 0.25 * K0 * math.pi / L
+        dk = kmax / 5000
+        clip = kmax / dk + 1
+        nbins = int(clip) + 1
+        sm = torch.zeros(nbins, dtype=torch.float64, device="cuda")
+        ct = torch.zeros(nbins, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        self.radial_plot(v.data_ptr(), dk, clip, nbins, sm.data_ptr(), ct.data_ptr())
+        self.sync()
+        s, c = sm.cpu().numpy(), ct.cpu().numpy()
+        nz = np.nonzero(c)[0]
+        m = int(nz[-1]) if len(nz) else 0
+        return [(i * dk, s[i] / c[i], c[i]) for i in range(1, m) if c[i] > 0]
 
     # ---- real-space Ewald ------------------------------------------------
     def real_space(self, w, x, eta, rc):

@@ -126,6 +126,16 @@ int sfac_accum_S2(void *h, double *acc_d);
 int sfac_accum_cross(void *ha, void *hb, double *acc_d);
 int sfac_radial(void *h, const double *val_d, int nbins, double dk, double *sum_d,
                 double *cnt_d);
+/* sfac_radial with plot.py's |k| shells (plot.py:9-32), for the array run.py
+ * saves (K0 x K1 x ldim): along each axis of that array index i stands for
+ * i - n when i >= n/2 (n = K0, K1 or ldim), k_d = 2 pi i / L_d with the box's
+ * axis lengths L6[0..2] (plot.py assumes a cubic box of side L), and point h
+ * goes to bin b = (int) min(|k| / dk, clip); points with b >= nbins are left
+ * out.  plot.py's own output takes kmax = K0 pi / (4 L), dk = kmax / 5000,
+ * clip = kmax / dk + 1, nbins = (int) clip + 1, and reports the non-empty
+ * bins 1 .. m - 1 (m the last non-empty one) as k = b dk, sum / count, count. */
+int sfac_radial_plot(void *h, const double *val_d, double dk, double clip, int nbins,
+                     double *sum_d, double *cnt_d);
 
 /* Real-space Ewald part and corrections (notes.tex:66-79, test.py:83,
  * 126-132), which with en/de1 for f(z) = exp(-pi^2 z/eta^2)/(pi z) give the

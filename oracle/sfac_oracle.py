@@ -481,3 +481,29 @@ def real_space(L6, w, x, eta, rc):
     return dict(E=E, vir=np.array([vir[0, 0], vir[1, 1], vir[2, 2], vir[1, 0], vir[2, 0],
                                    vir[2, 1]]),
                 dx=dx, phi=phi, E_self=E_self, E_bg=E_bg, vir_bg=E_bg / V)
+
+
+def plot_radial(f, L):
+    """|k|-shell averages of a (K0, K1, ldim) array, restating plot.py:9-32:
+    along each axis of the array (whatever its length n, ldim for z) index i
+    stands for i - n when i >= n // 2 (plot.py:3-7), k_d = 2 pi i / L with one
+    box length L for all axes, kmax = K0 pi / (4 L) and dk = kmax / 5000; a
+    point's bin is the integer part of |k| / dk clipped to kmax / dk + 1.
+    Returns (k, mean, count) of the non-empty bins from 1 to one below the
+    last non-empty bin (plot.py drops bin 0 and the last bin of its count)."""
+    f = np.asarray(f, dtype=np.float64)
+
+    def axis(n):
+        i = np.arange(n)
+        i = np.where(i >= n // 2, i - n, i)
+        return i * np.pi * 2 / L
+
+    kmax = 0.25 * f.shape[0] * np.pi / L
+    dk = kmax / 5000
+    kx, ky, kz = (axis(n) for n in f.shape)
+    s2 = (kx * kx)[:, None, None] + (ky * ky)[None, :, None] + (kz * kz)[None, None, :]
+    b = np.minimum(np.sqrt(s2) / dk, kmax / dk + 1).astype(int).ravel()
+    cnt = np.bincount(b)
+    tot = np.bincount(b, weights=f.ravel())
+    return [(i * dk, tot[i] / cnt[i], float(cnt[i]))
+            for i in range(1, len(cnt) - 1) if cnt[i] > 0]

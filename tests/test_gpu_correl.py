@@ -148,3 +148,73 @@ def test_accumulate_sk_over_xyz_frames(tmp_path):
     err = np.abs(acc - ref).max() / np.abs(ref).max()
     say("xyz frames", nfr, "rel err", err)
     assert nfr == 3 and err < 1e-12
+
+
+def test_radial_plot_matches_reference_plot_py():
+    """The device |k| shells in plot.py's convention (sfac_radial_plot via
+    Sfac.plot_spectrum) reproduce the reference's plot.py output stored in
+    tests/golden/radial_plot.json, on the array it was made from."""
+    import json
+    import os
+    from conftest import ROOT
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_radial_golden import make_input
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "radial_plot.json")))
+    shape = tuple(g["shape"])
+    f = make_input(shape, g["seed"]).real
+    L = g["L"]
+    s = ew().Sfac(np.array([L, L, L, 0., 0., 0.]), np.array([shape[0], shape[1], 2 * (shape[2] - 1)]),
+                  4)
+    rows = s.plot_spectrum(f)
+    say("plot rows", len(rows), "fixture", len(g["rows"]))
+    assert len(rows) == len(g["rows"])
+    for (k, m, c), (k0, m0, c0) in zip(rows, g["rows"]):
+        assert k == k0 and c == c0 and abs(m - m0) <= 1e-15 * abs(m0)
+
+
+def test_trr_frames_through_device_accumulation():
+    """gmx_ewald.py:65-107 on the committed .trr trajectory
+    (tests/golden/water16.trr): the handle is made from the first frame's
+    box (SofL), each frame's oxygens and hydrogens are spread as two
+    species, and |S_O|^2, |S_H|^2 and S_O conj(S_H) are summed over the
+    frames on the device; against the oracle's sums of the same frames."""
+    import os
+    from conftest import ROOT
+    from ewaldcorrel_amd.traj import read_trr
+    from oracle.sfac_oracle import SofL
+    frames = list(read_trr(os.path.join(ROOT, "tests", "golden", "water16.trr")))
+    L6w = SofL(frames[0]["box"])
+    Kw = (20, 21, 22)
+    Hw = Kw[0] * Kw[1] * (Kw[2] // 2 + 1)
+    hO = ew().Sfac(L6w, np.array(Kw), 4)
+    hH = ew().Sfac(L6w, np.array(Kw), 4)
+    oO, oH = OracleSfac(L6w, Kw, 4), OracleSfac(L6w, Kw, 4)
+    aO = torch.zeros(Hw, dtype=torch.float64, device="cuda")
+    aH = torch.zeros(Hw, dtype=torch.float64, device="cuda")
+    aOH = torch.zeros(2 * Hw, dtype=torch.float64, device="cuda")
+    rO = rH = rOH = 0
+    for fr in frames:
+        xO = fr["x"][0::3]
+        xH = np.concatenate([fr["x"][1::3], fr["x"][2::3]])
+        wO, wH = np.full(len(xO), -0.834), np.full(len(xH), 0.417)
+        hO.sfac(wO, xO)
+        hH.sfac(wH, xH)
+        torch.cuda.synchronize()
+        hO.accum_S2(aO.data_ptr())
+        hH.accum_S2(aH.data_ptr())
+        hO.accum_cross(hH, aOH.data_ptr())
+        oO.sfac(wO, xO)
+        oH.sfac(wH, xH)
+        SO, SH = oO.get_S().ravel(), oH.get_S().ravel()
+        rO = rO + np.abs(SO) ** 2
+        rH = rH + np.abs(SH) ** 2
+        rOH = rOH + SO * np.conj(SH)
+    hO.sync()
+    hH.sync()
+    gOH = aOH.cpu().numpy()
+    gOH = gOH[0::2] + 1j * gOH[1::2]
+    eO = np.abs(aO.cpu().numpy() - rO).max() / np.abs(rO).max()
+    eH = np.abs(aH.cpu().numpy() - rH).max() / np.abs(rH).max()
+    eOH = np.abs(gOH - rOH).max() / np.abs(rOH).max()
+    say("trr frames", len(frames), eO, eH, eOH)
+    assert len(frames) == 3 and max(eO, eH, eOH) <= 1e-12

@@ -2,12 +2,14 @@
 closed forms, independent brute-force restatements and exact invariants.
 CPU only."""
 import itertools
+import os
+import sys
 from math import erfc, pi
 
 import numpy as np
 import pytest
 
-from conftest import case_T, rel
+from conftest import ROOT, case_T, rel
 from oracle.sfac_oracle import (Cell, LofS, OracleSfac, SofL, bspl_deval, bspl_dft,
                                 bspl_eval, bspline_coeffs, ewald_rad)
 
@@ -318,3 +320,19 @@ def test_ewald_beta_independence():
         eself = -beta / np.sqrt(pi) * (q * q).sum()
         tot.append(er + ereal + eself)
     assert abs(tot[0] - tot[1]) < 2e-6 * abs(tot[0])
+
+
+def test_plot_radial_against_reference_fixture():
+    """oracle.plot_radial reproduces the reference's plot.py output stored in
+    tests/golden/radial_plot.json (written by make_radial_golden.py)."""
+    import json
+    from oracle.sfac_oracle import plot_radial
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_radial_golden import make_input
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "radial_plot.json")))
+    f = make_input(tuple(g["shape"]), g["seed"]).real
+    assert f.sum() == g["f_real_sum"] and f.ravel()[:8].tolist() == g["f_real_first"]
+    rows = plot_radial(f, g["L"])
+    assert len(rows) == len(g["rows"])
+    for (k, m, c), (k0, m0, c0) in zip(rows, g["rows"]):
+        assert k == k0 and c == c0 and abs(m - m0) <= 1e-15 * abs(m0)
