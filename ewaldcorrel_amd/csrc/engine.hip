@@ -777,6 +777,20 @@ void Engine::set_atoms_(int n, const double *w, const double *x, double *dx, dou
 
 int Engine::stage_blocks(Stage s) const { return stage_blocks_n(s, desc_.n); }
 
+bool Engine::uses_stage(Stage s) const {
+    switch (s) {
+        case ST_YZ_FWD:
+        case ST_YZ_INV: return slab();
+        case ST_ZR2C:
+        case ST_YF:
+        case ST_YI:
+        case ST_ZC2R: return !slab();
+        case ST_GATHER_TILE: return gtile();
+        case ST_GATHER_SORTED: return !gtile();
+        default: return true;
+    }
+}
+
 int Engine::stage_blocks_n(Stage s, int natoms) const {
     const FrameDesc &D = desc_;
     const int nrows = K[0] * K[1];
@@ -1749,10 +1763,12 @@ void Batch::apply_atoms_(int f) {
     gens_[f] = e->desc_gen();
 }
 
+// The launch sequence of a batch: every transform and gather path that one
+// of its frames takes (a frame has no workgroups in the launches of the other
+// paths), so each frame runs the kernels it runs alone.
 void Batch::refresh_variant_() {
     infl32_ = frames_[0]->desc().infl32;
-    slab_ = true;
-    gtile_ = true;
+    bool any_slab = false, any_lines = false, any_tile = false, any_sorted = false;
     gmz_ = 1;
     slab_ept_ = 0;
     xpath_ = 0;
@@ -1761,12 +1777,28 @@ void Batch::refresh_variant_() {
     for (const Engine *e : frames_) {
         zchunk_ = zchunk_ || e->desc().ntz > 1;
         gzchunk_ = gzchunk_ || e->desc().gntz > 1;
-        slab_ = slab_ && e->slab();
-        gtile_ = gtile_ && e->gtile();
-        gmz_ = std::max(gmz_, e->gmz());
-        slab_ept_ = std::max(slab_ept_, e->slab_ept());
+        if (e->slab()) {
+            any_slab = true;
+            slab_ept_ = std::max(slab_ept_, e->slab_ept());
+        } else {
+            any_lines = true;
+        }
+        if (e->gtile()) {
+            any_tile = true;
+            gmz_ = std::max(gmz_, e->gmz());
+        } else {
+            any_sorted = true;
+        }
         xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
     }
+    seq_.assign({ST_BIN, ST_SORT, ST_SPREAD});
+    if (any_slab) seq_.push_back(ST_YZ_FWD);
+    if (any_lines) seq_.insert(seq_.end(), {ST_ZR2C, ST_YF});
+    seq_.push_back(ST_XFUSED);
+    if (any_slab) seq_.push_back(ST_YZ_INV);
+    if (any_lines) seq_.insert(seq_.end(), {ST_YI, ST_ZC2R});
+    if (any_tile) seq_.push_back(ST_GATHER_TILE);
+    if (any_sorted) seq_.push_back(ST_GATHER_SORTED);
 }
 
 Batch::~Batch() {
@@ -1805,7 +1837,8 @@ void Batch::upload_(hipStream_t st) {
         int acc = 0;
         for (int f = 0; f < nf; ++f) {
             offs_[(size_t)s * (nf + 1) + f] = acc;
-            acc += frames_[f]->stage_blocks((Stage)s);
+            if (!frames_[f]->uses_stage((Stage)s)) continue;  // another path's launch
+            acc += frames_[f]->stage_blocks_n((Stage)s, atoms_[f].n);
             lds_[s] = std::max(lds_[s], frames_[f]->stage_lds((Stage)s, xpath_));
         }
         offs_[(size_t)s * (nf + 1) + nf] = acc;
@@ -1842,16 +1875,16 @@ void Batch::evaluate(hipStream_t st) {
     const int nf = (int)frames_.size();
     Engine *e0 = frames_[0];
     auto run_all = [&]() {
-        Stage prev = ST_COUNT;
-        const StageSeq seq = eval_stages(slab_, gtile_);
-        for (int i = 0; i < seq.n; ++i) {
-            const Stage s = seq.s[i];
+        for (const Stage s : seq_) {
             const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_,
                            zchunk_, gzchunk_, infl32_};
+            // the first launch after the fused x pass that holds a frame's
+            // workgroups finalizes its energy and virial (the slab or the y
+            // inverse, whichever path the frame takes)
+            const int finish = (s == ST_YZ_INV || s == ST_YI) ? 1 : 0;
             e0->prof_begin_on(stage_name(s), st);
-            e0->launch(s, L, total_[s], lds_[s], st, finishes(prev, s));
+            e0->launch(s, L, total_[s], lds_[s], st, finish);
             e0->prof_end_on(st);
-            prev = s;
         }
     };
     if (use_graph && !e0->profiling()) {

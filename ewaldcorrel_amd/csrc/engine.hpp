@@ -211,6 +211,9 @@ public:
     FrameDesc &desc_mut() { return desc_; }
     int stage_blocks(Stage s) const;
     int stage_blocks_n(Stage s, int natoms) const;  // for natoms atoms
+    // whether an evaluation of this engine launches stage s (the transform
+    // and gather stages belong to one path each)
+    bool uses_stage(Stage s) const;
     // per-atom sort workspace: bytes for n atoms, and its parts in D at base
     size_t workspace_bytes(size_t n) const;
     int bucket_cap(size_t n) const;  // records per plane bucket for n atoms
@@ -354,10 +357,9 @@ private:
     std::vector<int> total_;                // blocks per stage
     std::vector<size_t> lds_;               // max LDS per stage
     bool dirty_ = true;
-    bool slab_ = false;                     // every frame runs the slab transforms
-    bool gtile_ = false;                    // every frame has gather tiles
-    int gmz_ = 1;                           // largest row chunk count of the frames
-    int slab_ept_ = 0;                      // slab kernel variant (largest frame)
+    std::vector<Stage> seq_;                // launches of an evaluation (every frame's paths)
+    int gmz_ = 1;                           // largest row chunk count of the tiled frames
+    int slab_ept_ = 0;                      // slab kernel variant (largest slab frame)
     int xpath_ = 0;                         // x transform path valid for every frame
     int zchunk_ = 0;                        // some frame spreads through z-chunked tiles
     int gzchunk_ = 0;                       // some frame gathers through z-chunked tiles
