@@ -113,11 +113,12 @@ class Inputs:
 SET_A_MS = []  # wall time of each influence-table build (device builder + check)
 
 
-def make_handles(systems, precision, set_a_ms=None):
+def make_handles(systems, precision, set_a_ms=None, fp32_influence=False):
     from ewaldcorrel_amd import ewald
     hs = []
     for sy in systems:
-        s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision)
+        s = ewald.Sfac(sy.L6, np.array(sy.K), sy.order, precision=precision,
+                       fp32_influence=fp32_influence)
         t0 = time.perf_counter()
         s.set_A(ewald.ewald_kernel(sy.beta), sy.max_m)
         (SET_A_MS if set_a_ms is None else set_a_ms).append(1e3 * (time.perf_counter() - t0))
@@ -223,17 +224,19 @@ def time_streams(batches, streams, nframes, steps, warmup):
                 mean_event_ms=float(lat.mean()))
 
 
-def run_precision(systems, precision, steps, warmup, profile, tune=True, split=(2, 4)):
+def run_precision(systems, precision, steps, warmup, profile, tune=True, split=(2, 4),
+                  fp32_influence=False, single=True):
     """Single-frame evaluations with the default launch shapes, then the batch
     of all frames, autotuned first when `tune` (outside the timed region, as
     the reference's FFTW planning is)."""
     import torch
     from ewaldcorrel_amd import ewald
     dev = torch.device("cuda:0")
-    handles = make_handles(systems, precision)
+    handles = make_handles(systems, precision, None if not fp32_influence else [],
+                           fp32_influence=fp32_influence)
     inputs = [Inputs(sy, dev) for sy in systems]
     torch.cuda.synchronize()
-    single = time_single(systems[0], handles[0], inputs[0], steps, warmup)
+    single = time_single(systems[0], handles[0], inputs[0], steps, warmup) if single else None
     batch = ewald.SfacBatch(handles)
     for f, inp in enumerate(inputs):
         batch.set(f, inp.n, inp.w.data_ptr(), inp.x.data_ptr(), inp.out.data_ptr(),
@@ -472,24 +475,45 @@ def serve_stream(count, pool_size, rate, slots, max_batch, seed=5):
     return res
 
 
-def cpu_baseline(sy, budget_s=15.0):
-    """The FP64 oracle (numpy restatement of the reference algorithm) timed
-    on this host, one thread, on a bounded number of evaluations."""
-    from oracle.sfac_oracle import OracleSfac, ewald_rad
-    o = OracleSfac(sy.L6, sy.K, sy.order)
-    o.set_A(ewald_rad(sy.beta), sy.max_m)
-    o.evaluate(sy.w, sy.x)  # warm-up
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        o.evaluate(sy.w, sy.x)
-        k += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"{sy.name}: {k} full evaluations of one frame (FP64 numpy oracle, "
-                      f"single thread, {dt:.1f} s; set_A excluded)"}
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sy, E_gpu64=None):
+    """The reference's CPU path, restated in C++ (oracle/sfac_cpu.cpp: multimap
+    sort, gather-form OpenMP spreading, OpenMP /B and reductions, serial
+    de2/potl; the host has no FFTW, so its transforms are the port's own
+    single-threaded FFT), timed on this host on one frame: three evaluations
+    after a warm-up at the host's thread count (the value), one at a single
+    thread, and one of the -O0 build (the reference's Makefile builds -O0).
+    set_A is outside the timing, as in the reference's use."""
+    from oracle.cpu_port import run_port
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+    rep, (E, vir, dx, phi) = run_port(sy.L6, sy.K, sy.order, sy.beta, sy.max_m, sy.w, sy.x,
+                                      reps=3, threads=threads)
+    rep1, _ = run_port(sy.L6, sy.K, sy.order, sy.beta, sy.max_m, sy.w, sy.x, reps=1, threads=1)
+    rep0, _ = run_port(sy.L6, sy.K, sy.order, sy.beta, sy.max_m, sy.w, sy.x, reps=1,
+                       threads=threads, opt="O0")
+    steps = {k[:-2]: rep[k] for k in ("sort_s", "spread_s", "fft_fwd_s", "divB_s", "de1_s",
+                                      "de2_s", "potl_s")}
+    return {"value": 1.0 / rep["eval_s"], "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{sy.name} frame 0 ({sy.n} atoms, {sy.K[0]}^3, order {sy.order}): "
+                      f"median of 3 evaluations after a warm-up at {threads} OpenMP threads "
+                      "(C++ -O3 restatement of the reference CPU path, FP64; no FFTW on the "
+                      "host: the port's own single-threaded FFT)",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "seconds_per_eval": rep["eval_s"], "steps_s": steps,
+            "single_thread_evals_per_s": 1.0 / rep1["eval_s"],
+            "O0_evals_per_s": 1.0 / rep0["eval_s"],
+            "energy_rel_vs_gpu_fp64": (abs(E - E_gpu64) / abs(E_gpu64)
+                                       if E_gpu64 is not None else None)}
 
 
 # kernel symbol prefix -> stage name of the batched FP32 evaluation
@@ -575,7 +599,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--workload", default="water_98k")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (no autotuning)")
@@ -587,7 +610,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=8)
-    ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--serve-pool", type=int, default=200)
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="reduce two rocprofv3 counter passes of --counters to the "
                          "per-stage traffic file read by the roofline")
@@ -626,6 +649,24 @@ def main():
                              tune=not args.no_tune)
     log(f"fp32: single {s32['evals_per_s']:.0f} evals/s ({s32['p50_ms']:.3f} ms); "
         f"batch {b32['evals_per_s']:.0f} evals/s, p50 {b32['p50_ms']:.3f} ms/batch")
+    _, b32i = run_precision(systems, "fp32", args.steps, args.warmup, profile=True,
+                            tune=not args.no_tune, split=(), fp32_influence=True, single=False)
+    log(f"fp32 with the FP32 influence option: batch {b32i['evals_per_s']:.0f} evals/s, "
+        f"p50 {b32i['p50_ms']:.3f} ms/batch")
+
+    # frame 0 against the FP64 oracle
+    from oracle.sfac_oracle import OracleSfac, ewald_rad
+    o = OracleSfac(sy.L6, sy.K, sy.order)
+    o.set_A(ewald_rad(sy.beta), sy.max_m)
+    E0, v0, dx0, phi0 = o.evaluate(sy.w, sy.x)
+
+    def vs_oracle(b):
+        return {"energy_rel": abs(b["E"][0] - E0) / abs(E0),
+                "virial_rel_max": float(np.max(np.abs(b["vir"][0] - v0)) / np.max(np.abs(v0))),
+                "force_rel_rms": float(np.sqrt(np.mean((b["dx0"] - dx0) ** 2)) /
+                                       np.sqrt(np.mean(dx0 ** 2))),
+                "potential_rel_rms": float(np.sqrt(np.mean((b["phi0"] - phi0) ** 2)) /
+                                           np.sqrt(np.mean(phi0 ** 2)))}
 
     # FP32 against the FP64 engine on the same inputs
     eE = max(abs(a - b) / abs(b) for a, b in zip(b32["E"], b64["E"]))
@@ -636,22 +677,33 @@ def main():
     eVir = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
                for a, b in zip(b32["vir"], b64["vir"]))
 
-    # dominant kernel of the batched FP32 engine, timed live with HIP events
+    # The spreading stage of the batched FP32 engine (bin + sort + spread, one
+    # launch each), timed live with HIP events on the batch's stream, against
+    # the spreading bytes of one evaluation: N 4p (x, w) + G p (grid written
+    # once), p = 4, for every frame of the launch.
     B = args.batch
     ks = b32["kernels"]
-    dom = max(ks, key=lambda k: ks[k]["total_ms"])
-    nb = kernel_bytes(dom, sy.n, sy.K, 4)
-    Bk = nb * B if nb else None
-    avg_s = ks[dom]["avg_us"] * 1e-6
-    achieved = Bk / avg_s / 1e9 if Bk else None
+    stage = [k for k in ("bin", "sort", "spread") if k in ks]
+    stage_us = sum(ks[k]["avg_us"] for k in stage)
+    G = int(np.prod(sy.K))
+    Bk = (sy.n * 4 * 4 + G * 4) * B
+    achieved = Bk / (stage_us * 1e-6) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{sy.name}_fp32_batch{B}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(dom)
+            tr = json.load(f)
+        if all(k in tr for k in stage):
+            traffic = sum(tr[k] for k in stage)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": dom, "algorithmic_bytes": Bk, "avg_us": ks[dom]["avg_us"],
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": (os.path.relpath(tfile, ROOT) + " (rocprofv3 FETCH_SIZE + "
+                                   "WRITE_SIZE of the same launches, calibrated by a device "
+                                   "copy; a stored counter run, not measured in this run)")
+                if traffic is not None else None,
+                "kernel": "spreading stage: " + " + ".join(stage),
+                "algorithmic_bytes": Bk, "avg_us": stage_us,
+                "kernels_avg_us": {k: ks[k]["avg_us"] for k in stage},
                 "frames_per_launch": B}
     Be32 = eval_bytes(sy.n, sy.K, 4, 4) * B  # the FP32 grid is written once, as FP32
     Be64 = eval_bytes(sy.n, sy.K, 8, 8) * B
@@ -677,9 +729,11 @@ def main():
             serving["poisson"]["req_per_s"], serving["poisson"]["latency_ms"]["p50"],
             serving["poisson"]["latency_ms"]["p99"], serving["saturated"]["req_per_s"]))
 
-    cpu = None if args.no_cpu else cpu_baseline(sy, args.cpu_budget)
+    cpu = None if args.no_cpu else cpu_baseline(sy, b64["E"][0])
     if cpu:
-        log(f"cpu baseline: {cpu['value']:.3f} evals/s ({cpu['sample']})")
+        log(f"cpu baseline: {cpu['value']:.3f} evals/s at {cpu['cores']} threads "
+            f"({cpu['single_thread_evals_per_s']:.3f} at 1, -O0 {cpu['O0_evals_per_s']:.3f}); "
+            f"{cpu['cpu_model']}")
 
     line = {
         "metric": METRIC,
@@ -696,8 +750,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": sy.name, "atoms": sy.n, "grid": list(sy.K), "order": sy.order,
                    "beta": sy.beta, "max_m": sy.max_m, "frames_per_step": B,
-                   "precision": "fp32 (FP64 coordinates and reductions, int64 fixed-point "
-                                "grid accumulation)",
+                   "precision": "fp32 spreading, FFTs and gather; FP64 coordinates, /B, "
+                                "influence, energy/virial sums and reductions; int64 "
+                                "fixed-point grid accumulation",
                    "influence": ("closed form from the indices (no table read)"
                                  if b32["table_mode"] == 1 else "table read from HBM")},
         "set_A_ms": {"median": float(np.median(SET_A_MS)), "max": float(np.max(SET_A_MS)),
@@ -718,11 +773,21 @@ def main():
                  "dtype": "f64",
                  "eval_roofline_frac": Be64 / (b64["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "speedup_fp32_vs_fp64": b32["evals_per_s"] / b64["evals_per_s"],
+        "speedup_fp32_batch_vs_fp64_single": b32["evals_per_s"] / s64["evals_per_s"],
+        "fp32_error_vs_fp64_oracle": dict(vs_oracle(b32), over="frame 0"),
+        "fp64_error_vs_fp64_oracle": dict(vs_oracle(b64), over="frame 0"),
         "fp32_error_vs_fp64_engine": {"energy_rel_max": eE, "virial_rel_max": eVir,
                                       "force_rel_rms": eF, "potential_rel_rms": ePhi,
                                       "over": "every frame's energy and virial (max "
                                               "component error over the largest), frame "
                                               "0's forces and potentials"},
+        "fp32_influence_option": {
+            "value": b32i["evals_per_s"], "ms_per_step": b32i["ms_per_step"],
+            "latency_ms": {"p50": b32i["p50_ms"], "p99": b32i["p99_ms"], "per": "batch"},
+            "what": "SFAC_FP32_INFLUENCE: the closed-form influence and the per-thread "
+                    "energy/virial sums in FP32 (the rest as the FP32 engine)",
+            "error_vs_fp64_oracle": dict(vs_oracle(b32i), over="frame 0"),
+            "kernels_avg_us": {k: round(v["avg_us"], 3) for k, v in b32i["kernels"].items()}},
         "eval_roofline": {"algorithmic_bytes_per_step": Be32,
                           "achieved_GBs": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9,
                           "frac": Be32 / (b32["mean_event_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
