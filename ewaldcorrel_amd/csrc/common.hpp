@@ -113,6 +113,12 @@ template <> struct __align__(16) AtomRec<double> {
 static_assert(sizeof(AtomRec<float>) == 32, "AtomRec<float> is 32 bytes");
 static_assert(sizeof(AtomRec<double>) == 48, "AtomRec<double> is 48 bytes");
 
+// a box whose x axis separates from y and z in reciprocal space (iL10 = iL20
+// = 0): m_x depends on kx only and m_y, m_z on (ky, kz) only
+__host__ __device__ inline bool x_separable(const CellArgs &cl) {
+    return cl.iL10 == 0.0 && cl.iL20 == 0.0;
+}
+
 struct FrameDesc {
     // geometry
     int K0, K1, K2, ldim, order;

@@ -107,26 +107,33 @@ template <typename T> SplineArgs<T> spline_args(const std::vector<double> &M, in
     return s;
 }
 
+template <typename T, int MODE, int INF>
+void launch_x_inf(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int sign,
+                  int finish) {
+    if (L.xpath == kXBlock)
+        fft_x_kernel<T, MODE, kXBlock, INF><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
+                                                                finish);
+    else
+        fft_x_kernel<T, MODE, kXGeneric, INF><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
+                                                                  finish);
+}
+
 template <typename T, int MODE>
 void launch_x(const Launch &L, dim3 g, dim3 bl, size_t lds, hipStream_t st, int sign,
               int finish) {
-    // only the fused pass (MODE 2) applies the influence; the FP32 influence
-    // option of the FP32 engine has its own instantiation
-    if constexpr (MODE == 2 && std::is_same<T, float>::value) {
-        if (L.infl32) {
-        if (L.xpath == kXBlock)
-            fft_x_kernel<T, MODE, kXBlock, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
-        else
-            fft_x_kernel<T, MODE, kXGeneric, 1><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
-                                                                   finish);
-        return;
+    // only the fused pass (MODE 2) applies the influence, in the form the
+    // launch's frames share (XInfluence), each its own instantiation
+    if constexpr (MODE == 2) {
+        if (std::is_same<T, float>::value && L.xinf == kXInfFp32) {
+            launch_x_inf<T, MODE, kXInfFp32>(L, g, bl, lds, st, sign, finish);
+            return;
+        }
+        if (L.xinf == kXInfSeparable) {
+            launch_x_inf<T, MODE, kXInfSeparable>(L, g, bl, lds, st, sign, finish);
+            return;
         }
     }
-    if (L.xpath == kXBlock)
-        fft_x_kernel<T, MODE, kXBlock, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign, finish);
-    else
-        fft_x_kernel<T, MODE, kXGeneric, 0><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, sign,
-                                                               finish);
+    launch_x_inf<T, MODE, kXInfAny>(L, g, bl, lds, st, sign, finish);
 }
 
 template <typename T, int ORDER, int MZ>
@@ -847,7 +854,7 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
         case ST_XFUSED:  // the lines, the wavefront sums, the closed form's x factors
             return (fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
                          : xlines_lds_bytes<double>(K[0], D.xC, xgen)) +
-                   kXWaveSums + (size_t)((K[0] + 1) & ~1) * sizeof(double);
+                   xfused_extra_lds(K[0], D.xC);
         case ST_YZ_FWD:
         case ST_YZ_INV:
             return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
@@ -880,7 +887,7 @@ void Engine::run_(Stage s, int finish) {
     sync_desc_();
     ProfScope ps(this, stage_name(s));
     launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
-                     desc_.gntz > 1, desc_.infl32}, nb,
+                     desc_.gntz > 1, x_influence_form(desc_)}, nb,
            stage_lds(s), stream_,
            finish);
 }
@@ -1019,7 +1026,7 @@ void Engine::evaluate_noalloc(int n, const double *w_d, const double *x_d, doubl
     HIPC(hipMemcpyAsync(d_desc_na_, &desc_na_, sizeof(FrameDesc), hipMemcpyHostToDevice,
                         stream_));
     const Launch L{d_desc_na_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
-                   desc_.gntz > 1, desc_.infl32};
+                   desc_.gntz > 1, x_influence_form(desc_)};
     const StageSeq seq = eval_stages(slab(), gtile());
     Stage prev = ST_COUNT;
     for (int i = 0; i < seq.n; ++i) {
@@ -1158,7 +1165,7 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(gmz() & 0xffff) << 32;
     v |= (unsigned long long)(desc_.ntz > 1 ? 1 : 0) << 48;
     v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
-    v |= (unsigned long long)(desc_.infl32 ? 1 : 0) << 50;
+    v |= (unsigned long long)(x_influence_form(desc_) & 3) << 50;
     return v;
 }
 
@@ -1767,7 +1774,13 @@ void Batch::apply_atoms_(int f) {
 // of its frames takes (a frame has no workgroups in the launches of the other
 // paths), so each frame runs the kernels it runs alone.
 void Batch::refresh_variant_() {
-    infl32_ = frames_[0]->desc().infl32;
+    // the x-separable form only if every frame takes it (a frame's results
+    // are the same in either kernel)
+    xinf_ = kXInfSeparable;
+    for (const Engine *e : frames_) {
+        const int f = x_influence_form(e->desc());
+        if (f != kXInfSeparable) xinf_ = f;
+    }
     bool any_slab = false, any_lines = false, any_tile = false, any_sorted = false;
     gmz_ = 1;
     slab_ept_ = 0;
@@ -1877,7 +1890,7 @@ void Batch::evaluate(hipStream_t st) {
     auto run_all = [&]() {
         for (const Stage s : seq_) {
             const Launch L{d_descs_, d_offs_ + (size_t)s * (nf + 1), nf, slab_ept_, xpath_, gmz_,
-                           zchunk_, gzchunk_, infl32_};
+                           zchunk_, gzchunk_, xinf_};
             // the first launch after the fused x pass that holds a frame's
             // workgroups finalizes its energy and virial (the slab or the y
             // inverse, whichever path the frame takes)

@@ -56,8 +56,17 @@ struct Launch {
     int gmz;       // row chunks of 64 per lane of the tiled gather (ceil((K2 + n - 1) / 64))
     int zchunk;    // some frame spreads through z-chunked tiles
     int gzchunk;   // some frame gathers through z-chunked tiles
-    int infl32;    // the frames use the FP32 influence option (FP32 engine)
+    int xinf;      // influence form of the fused x pass the frames share (XInfluence)
 };
+
+// Influence form of a fused x launch: kXInfAny (FP64: table, or the closed
+// form of any box), kXInfFp32 (the FP32 engine's FP32 influence option),
+// kXInfSeparable (FP64 closed form, boxes whose x axis separates).
+enum XInfluence { kXInfAny = 0, kXInfFp32 = 1, kXInfSeparable = 2 };
+inline int x_influence_form(const FrameDesc &D) {
+    if (D.infl32) return kXInfFp32;
+    return D.tabmode == kTableAnalytic && x_separable(D.tcell) ? kXInfSeparable : kXInfAny;
+}
 
 struct FftPlan {
     int N = 0;
@@ -363,7 +372,7 @@ private:
     int xpath_ = 0;                         // x transform path valid for every frame
     int zchunk_ = 0;                        // some frame spreads through z-chunked tiles
     int gzchunk_ = 0;                       // some frame gathers through z-chunked tiles
-    int infl32_ = 0;                        // the frames use the FP32 influence option
+    int xinf_ = 0;                          // influence form of the fused x launches
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
 };

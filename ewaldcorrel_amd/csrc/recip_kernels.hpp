@@ -452,6 +452,16 @@ __host__ __device__ inline int x_path(const FrameDesc &F) {
 // transform, so that no sums are held in registers through the inverse.
 constexpr size_t kXWaveSums = (kBlock / 64) * 8 * sizeof(double);
 
+// The rest of the fused x pass's LDS, after the wavefront sums, for the
+// closed form of x-separable boxes: gx[|nx|] = e^(fac iL00^2 nx^2) for
+// |nx| <= N/2 (padded to even) and kXColConsts constants per column (the
+// line's class constants, exponentials, 1/B of y and z and multiplicity).
+constexpr int kXColConsts = 6;
+__host__ __device__ inline size_t xfused_gx_doubles(int N) { return (size_t)((N / 2 + 2) & ~1); }
+__host__ __device__ inline size_t xfused_extra_lds(int N, int C) {
+    return kXWaveSums + (xfused_gx_doubles(N) + (size_t)kXColConsts * C) * sizeof(double);
+}
+
 // The influence step of the fused x pass on the transformed lines in LDS:
 // S-hat = S iB0 iB1 iB2, the energy and virial sums of A |S-hat|^2 (thread
 // sums in acc[7]) and S-hat A iB (x2 on the kz = 0 and Nyquist planes, which
@@ -541,13 +551,16 @@ __device__ __forceinline__ void x_influence(const FrameDesc &F, cplx<T> *res,
 //     Wyz = sum w sy sz, folded once per thread:
 //       v_ab = W2 a_a a_b + Wy (a_a u_b + a_b u_a) + Wz (a_a z_b + a_b z_a)
 //              + W0 (u_a u_b + z_a z_b) + Wyz (u_a z_b + z_a u_b).
-//   * Orthorhombic boxes (iL10 = iL20 = 0): e^(fac m^2) = gx[kx] e^(fac |b|^2)
-//     (b = sy u + sz z), gx[kx] = e^(fac iL00^2 nx^2) formed once per
-//     workgroup in LDS; other boxes take e^(fac m^2) per image.  1/m^2 by the
-//     hardware reciprocal and two Newton steps.
-//   * m^2 is formed with contractions; within a relative 1e-12 of the cutoff
-//     the ball test is redone exactly as ewald_image computes m, so the set
-//     of images is the table's.
+//   * m^2 = q00 nx^2 + 2 nx c1 + c0 with the class constants c0 = |b|^2,
+//     c1 = a.b of b = sy u + sz z; it is formed with contractions, and within
+//     a relative 1e-12 of the cutoff the ball test is redone exactly as
+//     ewald_image computes m, so the set of images is the table's.
+//   * 1/m^2 by the hardware reciprocal and two Newton steps.
+// Boxes whose x axis separates (iL10 = iL20 = 0, x_influence64_xsep) have
+// c1 = 0 and e^(fac m^2) = gx[kx] e^(fac |b|^2), gx[kx] = e^(fac iL00^2 nx^2)
+// formed once per workgroup in LDS; |b|^2 depends on the class only through
+// sy sz, so each thread takes at most two exponentials for its whole line and
+// none per point.  Other boxes (x_influence64_gen) take e^(fac m^2) per image.
 // The wavefront sums of E and the virial go to wsums[wavefront][0..6].
 __device__ __forceinline__ double recip_f64(double x) {
     double r = __builtin_amdgcn_rcp(x);
@@ -555,41 +568,30 @@ __device__ __forceinline__ double recip_f64(double x) {
     return fma(fma(-x, r, 1.0), r, r);
 }
 
-template <typename T>
-__device__ __forceinline__ void x_influence64(const FrameDesc &F, cplx<T> *res,
-                                              const double *ib0, const double *gx, int LS,
-                                              int c, int e0, int es, bool valid, int ky, int kz,
-                                              double ibyz, double *wsums) {
-    constexpr int EPT = kXElems / kBlock;
-    const int N = F.plan0.N, K0 = F.K0, K1 = F.K1, K2 = F.K2;
-    const CellArgs &cl = F.tcell;
-    const double fac = F.ewald_fac, max_m2 = F.max_m2;
-    const bool ortho = cl.iL10 == 0.0 && cl.iL20 == 0.0;
-    const double q00 = cl.iL00 * cl.iL00 + cl.iL10 * cl.iL10 + cl.iL20 * cl.iL20;
-    const int ny = ky <= K1 / 2 ? ky : ky - K1;
-    const bool yny = (K1 % 2) == 0 && ky == K1 / 2;
-    const bool zny = (K2 % 2) == 0 && kz == K2 / 2;
-    const bool zn2 = zny && kz > 0;
-    const double mult = (kz == 0 || zny) ? 1.0 : 2.0;
-    // b = sy u + sz z of image class (sy, sz): its y, z components, |b|^2, a.b
-    auto bvec = [&](int sy, int sz, double &B1, double &B2) {
+// Line constants and the per-image accumulation shared by both forms.
+struct XLine64 {
+    int ny;
+    bool yny, zn2;
+    double mult;
+    double W0 = 0.0, W2 = 0.0, Wy = 0.0, Wz = 0.0, Wyz = 0.0, aE = 0.0;
+    __device__ __forceinline__ XLine64(const FrameDesc &F, int ky, int kz) {
+        ny = ky <= F.K1 / 2 ? ky : ky - F.K1;
+        yny = (F.K1 % 2) == 0 && ky == F.K1 / 2;
+        const bool zny = (F.K2 % 2) == 0 && kz == F.K2 / 2;
+        zn2 = zny && kz > 0;
+        mult = (kz == 0 || zny) ? 1.0 : 2.0;
+    }
+    // b = sy u + sz z of image class (sy, sz): its y and z components
+    __device__ __forceinline__ void bvec(const CellArgs &cl, int kz, int sy, int sz, double &B1,
+                                         double &B2) const {
         const double nyd = (double)(sy * ny), nzd = (double)(sz * kz);
         B1 = nyd * cl.iL11;
         B2 = fma(nyd, cl.iL21, nzd * cl.iL22);
-    };
-    // the line's own class (+, +)
-    double c0, c1, el;
-    {
-        double B1, B2;
-        bvec(1, 1, B1, B2);
-        c0 = fma(B1, B1, B2 * B2);
-        c1 = fma(cl.iL10, B1, cl.iL20 * B2);
-        el = ortho ? exp_nonpos(fac * c0) * (1.0 / M_PI) : 0.0;
     }
-    double W0 = 0.0, W2 = 0.0, Wy = 0.0, Wz = 0.0, Wyz = 0.0, aE = 0.0;
-    // m^2 = q00 nx^2 + 2 nx c1 + c0 of image nx (class constants c0, c1) and
-    // its ball test (redone exactly near the cutoff)
-    auto in_ball = [&](int nx, int sy, int sz, double cc0, double cc1, double &m2) {
+    // the ball test of image nx of class (sy, sz), m^2 returned
+    __device__ __forceinline__ bool in_ball(const CellArgs &cl, double q00, double max_m2,
+                                            int nx, int sy, int sz, int kz, double cc0,
+                                            double cc1, double &m2) const {
         const double nxd = nx;
         m2 = fma(fma(q00, nxd, 2.0 * cc1), nxd, cc0);
         bool in = !(m2 < 1e-10 || m2 > max_m2);
@@ -599,9 +601,10 @@ __device__ __forceinline__ void x_influence64(const FrameDesc &F, cplx<T> *res,
             in = !(m2e < 1e-10 || m2e > max_m2);
         }
         return in;
-    };
-    // the terms of one image: f returned, moments added
-    auto add_image = [&](int nx, int sy, int sz, double m2, double ex, double s) {
+    }
+    // the terms of one image (e^(fac m^2) / pi given): f returned, moments added
+    __device__ __forceinline__ double add_image(double fac, int nx, int sy, int sz, double m2,
+                                                double ex, double s) {
         const double r = recip_f64(m2);
         const double f = ex * r;
         const double w = mult * s * ((fac - r) * f);
@@ -613,8 +616,166 @@ __device__ __forceinline__ void x_influence64(const FrameDesc &F, cplx<T> *res,
         Wz = sz > 0 ? Wz + t : Wz - t;
         Wyz = sy * sz > 0 ? Wyz + w : Wyz - w;
         return f;
-    };
-    const bool nyq_line = yny || zn2;
+    }
+    // image k of a Nyquist point: its signed x index and class
+    __device__ __forceinline__ void nyq_image(int k, bool xny, int nx, int &nxs, int &sy,
+                                              int &sz) const {
+        int kk = k;
+        nxs = nx;
+        sy = 1;
+        sz = 1;
+        if (xny) {
+            if (kk & 1) nxs = -nx;
+            kk >>= 1;
+        }
+        if (yny) {
+            if (kk & 1) sy = -1;
+            kk >>= 1;
+        }
+        if (zn2 && (kk & 1)) sz = -1;
+    }
+    // fold of the sums into E and the virial, then the wavefront sums:
+    // a = (iL00, iL10, iL20), u = (0, uy, uz), z = (0, 0, zz)
+    __device__ __forceinline__ void finish(const CellArgs &cl, int kz, double *wsums) const {
+        const double uy = (double)ny * cl.iL11, uz = (double)ny * cl.iL21;
+        const double zz = (double)kz * cl.iL22;
+        const double a0 = cl.iL00, a1 = cl.iL10, a2 = cl.iL20;
+        double vv[7];
+        vv[0] = 0.5 * mult * aE;
+        vv[1] = W2 * a0 * a0;                                                // xx
+        vv[2] = W2 * a1 * a1 + 2.0 * Wy * a1 * uy + W0 * uy * uy;            // yy
+        vv[3] = W2 * a2 * a2 + 2.0 * (Wy * a2 * uz + Wz * a2 * zz) +
+                W0 * (uz * uz + zz * zz) + 2.0 * Wyz * uz * zz;              // zz
+        vv[4] = W2 * a1 * a0 + Wy * a0 * uy;                                 // yx
+        vv[5] = W2 * a2 * a0 + Wy * a0 * uz + Wz * a0 * zz;                  // zx
+        vv[6] = W2 * a2 * a1 + Wy * (a2 * uy + a1 * uz) + Wz * a1 * zz +
+                W0 * uy * uz + Wyz * uy * zz;                                // zy
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const double vs = wave_sum(vv[k]);
+            if (lane == 0) wsums[wv * 8 + k] = vs;
+        }
+    }
+};
+
+// x-separable boxes (iL10 = iL20 = 0).  Every image of a point has the
+// same m^2 within its group (the classes with sy sz = g, g = +-1: b of one
+// is the exact negation of the other's, c1 = 0, and nx -> -nx leaves m^2
+// unchanged), so each point evaluates one image per group and adds it with
+// its multiplicity and the group's sign sums: images nxs sy sz with nxs in
+// {nx} (or {nx, -nx} on the x Nyquist plane, xf = 2) and the line's pairs
+// (sy, sz) of the group (np_g pairs, sign sums Sy_g, Sz_g):
+//   a += xf np_g f,  W0 += xf np_g w,  W2 += xf np_g w nx^2,
+//   Wy += w (sum nxs) Sy_g,  Wz += w (sum nxs) Sz_g,  Wyz += xf g np_g w.
+// The column constants of the x-separable form (one thread per column):
+// k[0] = c0 = |b|^2 of group +, k[1] = cm of group - (c0 on lines that have
+// no group -), k[2], k[3] their e^(fac |b|^2) / pi, k[4] = iB1[ky] iB2[kz],
+// k[5] = mult.
+__device__ __forceinline__ void xsep_column(const FrameDesc &F, int ky, int kz, double *k) {
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac;
+    const XLine64 L(F, ky, kz);
+    double B1, B2;
+    L.bvec(cl, kz, 1, 1, B1, B2);
+    const double c0 = fma(B1, B1, B2 * B2);
+    const double el = exp_nonpos(fac * c0) * (1.0 / M_PI);
+    double cm = c0, elm = el;
+    if (L.yny || L.zn2) {
+        L.bvec(cl, kz, -1, 1, B1, B2);
+        cm = fma(B1, B1, B2 * B2);
+        elm = exp_nonpos(fac * cm) * (1.0 / M_PI);
+    }
+    k[0] = c0;
+    k[1] = cm;
+    k[2] = el;
+    k[3] = elm;
+    k[4] = F.iB1[ky] * F.iB2[kz];
+    k[5] = L.mult;
+}
+
+template <typename T>
+__device__ __forceinline__ void x_influence64_xsep(const FrameDesc &F, cplx<T> *res,
+                                                   const double *ib0, const double *gx,
+                                                   const double *colk, int LS, int c, int e0,
+                                                   int es, bool valid, int ky, int kz,
+                                                   double *wsums) {
+    constexpr int EPT = kXElems / kBlock;
+    const int N = F.plan0.N, K0 = F.K0;
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac, max_m2 = F.max_m2;
+    const double q00 = cl.iL00 * cl.iL00 + cl.iL10 * cl.iL10 + cl.iL20 * cl.iL20;
+    XLine64 L(F, ky, kz);
+    // pairs of the groups: no Nyquist line: (+,+); y only: (+,+) | (-,+);
+    // z only: (+,+) | (+,-); both: (+,+),(-,-) | (+,-),(-,+)
+    const bool both = L.yny && L.zn2;
+    const int ng = (L.yny || L.zn2) ? 2 : 1;
+    const double *k = colk + kXColConsts * c;
+#pragma unroll 1
+    for (int q = 0; q < EPT; ++q) {
+        const int e = e0 + q * es;
+        if (e >= N || !valid) continue;
+        const double ib = ib0[e] * k[4];
+        cplx<T> v = res[c * LS + e];
+        const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+        const double s = re * re + im * im;
+        const int nx = e <= K0 / 2 ? e : e - K0;
+        const bool xny = (K0 % 2) == 0 && e == K0 / 2;
+        double a = 0.0;
+#pragma unroll 1
+        for (int gi = 0; gi < ng; ++gi) {
+            // group g: the image checked is (nx, sy, sz) = (nx, 1, 1), or for
+            // g = -1 (nx, -1, 1) on a y Nyquist line, else (nx, 1, -1)
+            const int g = gi ? -1 : 1;
+            const int sy = L.yny ? g : 1, sz = L.yny ? 1 : g;
+            double m2;
+            if (!L.in_ball(cl, q00, max_m2, nx, sy, sz, kz, k[gi], 0.0, m2)) continue;
+            const double r = recip_f64(m2);
+            const double f = gx[nx < 0 ? -nx : nx] * k[2 + gi] * r;
+            const double w = k[5] * s * ((fac - r) * f);
+            const double nxd = nx;
+            // multiplicity xf np of the group's images, and w times the sums
+            // of nxs sy and nxs sz over them
+            const int mul = (xny ? 2 : 1) * (both ? 2 : 1);
+            const int ssy = (xny || both) ? 0 : nx * sy, ssz = (xny || both) ? 0 : nx * sz;
+            const double mw = (double)mul * w;
+            L.W0 += mw;
+            L.W2 = fma(mw * nxd, nxd, L.W2);
+            L.Wy = fma(w, (double)ssy, L.Wy);
+            L.Wz = fma(w, (double)ssz, L.Wz);
+            L.Wyz = gi ? L.Wyz - mw : L.Wyz + mw;
+            a = fma((double)mul, f, a);
+        }
+        L.aE = fma(a, s, L.aE);
+        const double tt = a * ib;
+        v.x = T(re * tt);
+        v.y = T(im * tt);
+        res[c * LS + e] = v;
+    }
+    L.finish(cl, kz, wsums);
+}
+
+// any box: e^(fac m^2) per image
+template <typename T>
+__device__ __forceinline__ void x_influence64_gen(const FrameDesc &F, cplx<T> *res,
+                                                  const double *ib0, int LS, int c, int e0,
+                                                  int es, bool valid, int ky, int kz,
+                                                  double ibyz, double *wsums) {
+    constexpr int EPT = kXElems / kBlock;
+    const int N = F.plan0.N, K0 = F.K0;
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac, max_m2 = F.max_m2;
+    const double q00 = cl.iL00 * cl.iL00 + cl.iL10 * cl.iL10 + cl.iL20 * cl.iL20;
+    XLine64 L(F, ky, kz);
+    // the line's own class (+, +)
+    double c0, c1;
+    {
+        double B1, B2;
+        L.bvec(cl, kz, 1, 1, B1, B2);
+        c0 = fma(B1, B1, B2 * B2);
+        c1 = fma(cl.iL10, B1, cl.iL20 * B2);
+    }
+    const bool nyq_line = L.yny || L.zn2;
 #pragma unroll 1
     for (int q = 0; q < EPT; ++q) {
         const int e = e0 + q * es;
@@ -629,82 +790,54 @@ __device__ __forceinline__ void x_influence64(const FrameDesc &F, cplx<T> *res,
         if (!(xny || nyq_line)) {
             // one image, the line's own class
             double m2;
-            if (in_ball(nx, 1, 1, c0, c1, m2))
-                a = add_image(nx, 1, 1, m2,
-                              ortho ? gx[e] * el : exp_nonpos(m2 * fac) * (1.0 / M_PI), s);
+            if (L.in_ball(cl, q00, max_m2, nx, 1, 1, kz, c0, c1, m2))
+                a = L.add_image(fac, nx, 1, 1, m2, exp_nonpos(m2 * fac) * (1.0 / M_PI), s);
         } else {
             // Nyquist planes: every signed image, its class constants anew
-            const int nimg = (xny ? 2 : 1) * (yny ? 2 : 1) * (zn2 ? 2 : 1);
+            const int nimg = (xny ? 2 : 1) * (L.yny ? 2 : 1) * (L.zn2 ? 2 : 1);
 #pragma unroll 1
             for (int k = 0; k < nimg; ++k) {
-                int kk = k, nxs = nx, sy = 1, sz = 1;
-                if (xny) {
-                    if (kk & 1) nxs = -nx;
-                    kk >>= 1;
-                }
-                if (yny) {
-                    if (kk & 1) sy = -1;
-                    kk >>= 1;
-                }
-                if (zn2 && (kk & 1)) sz = -1;
+                int nxs, sy, sz;
+                L.nyq_image(k, xny, nx, nxs, sy, sz);
                 double B1, B2, m2;
-                bvec(sy, sz, B1, B2);
+                L.bvec(cl, kz, sy, sz, B1, B2);
                 const double cc0 = fma(B1, B1, B2 * B2);
                 const double cc1 = fma(cl.iL10, B1, cl.iL20 * B2);
-                if (!in_ball(nxs, sy, sz, cc0, cc1, m2)) continue;
-                const double ex = ortho ? gx[e] * exp_nonpos(fac * cc0) * (1.0 / M_PI)
-                                        : exp_nonpos(m2 * fac) * (1.0 / M_PI);
-                a += add_image(nxs, sy, sz, m2, ex, s);
+                if (!L.in_ball(cl, q00, max_m2, nxs, sy, sz, kz, cc0, cc1, m2)) continue;
+                a += L.add_image(fac, nxs, sy, sz, m2, exp_nonpos(m2 * fac) * (1.0 / M_PI), s);
             }
         }
-        aE = fma(a, s, aE);
+        L.aE = fma(a, s, L.aE);
         const double tt = a * ib;
         v.x = T(re * tt);
         v.y = T(im * tt);
         res[c * LS + e] = v;
     }
-    // fold: a = (iL00, iL10, iL20), u = (0, uy, uz), z = (0, 0, zz)
-    const double uy = (double)ny * cl.iL11, uz = (double)ny * cl.iL21;
-    const double zz = (double)kz * cl.iL22;
-    const double a0 = cl.iL00, a1 = cl.iL10, a2 = cl.iL20;
-    double vv[7];
-    vv[0] = 0.5 * mult * aE;
-    vv[1] = W2 * a0 * a0;                                                // xx
-    vv[2] = W2 * a1 * a1 + 2.0 * Wy * a1 * uy + W0 * uy * uy;            // yy
-    vv[3] = W2 * a2 * a2 + 2.0 * (Wy * a2 * uz + Wz * a2 * zz) +
-            W0 * (uz * uz + zz * zz) + 2.0 * Wyz * uz * zz;              // zz
-    vv[4] = W2 * a1 * a0 + Wy * a0 * uy;                                 // yx
-    vv[5] = W2 * a2 * a0 + Wy * a0 * uz + Wz * a0 * zz;                  // zx
-    vv[6] = W2 * a2 * a1 + Wy * (a2 * uy + a1 * uz) + Wz * a1 * zz +
-            W0 * uy * uz + Wyz * uy * zz;                                // zy
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        const double vs = wave_sum(vv[k]);
-        if (lane == 0) wsums[wv * 8 + k] = vs;
-    }
+    L.finish(cl, kz, wsums);
 }
 
 // The influence of the fused x pass is a template choice (INF): 0 the FP64
-// forms (table or closed form), 1 the FP32 closed form of the FP32 engine's
-// option (table frames still take the FP64 table path).  Each is its own
-// instantiation, so that neither holds the other's registers.
-// Waves per SIMD the fused x launches are held to (registers): FP32 with the
-// FP32 influence at 64 VGPRs (line transforms with per-stage indices) runs
-// eight workgroups per CU; with the FP64 influence five (its FP64 per-point
-// terms need ~96 VGPRs: at six or more they spill inside the point loop, and
-// five and four measured the same, 141 us per 16-frame water launch, so the
-// pass is bound by its FP64 arithmetic there); FP64 at 102 would spill.
+// forms for any frame (table, or the closed form of either box class), 1 the
+// FP32 closed form of the FP32 engine's option (table frames still take the
+// FP64 table path), 2 the FP64 closed form of x-separable boxes only.  Each is
+// its own instantiation, so that none holds another's registers; a frame's
+// results do not depend on which of 0 and 2 it runs in (same arithmetic).
+// Waves per SIMD the fused x launches are held to (registers): FP32 at 64
+// VGPRs (line transforms with per-stage indices) runs eight workgroups per
+// CU, with the FP32 influence and with the FP64 x-separable one; the general
+// FP64 form (a per-image exponential) needs ~96 VGPRs (at six or more waves
+// it spills inside the point loop), five waves.
 constexpr int kXWaves64 = 5;
 template <typename T, int MODE> constexpr bool x_lean() {
     return MODE == 2 && std::is_same<T, float>::value;
 }
-template <typename T, int MODE, int INF> constexpr int x_waves() {
-    return x_lean<T, MODE>() ? (INF ? 8 : kXWaves64) : 1;
+template <typename T, int MODE, int INF, int XP> constexpr int x_waves() {
+    // the ping-pong transform of generic radices holds a little more
+    return x_lean<T, MODE>() ? (INF == 0 ? kXWaves64 : (INF == 2 && XP == 1) ? 7 : 8) : 1;
 }
 
 template <typename T, int MODE, int XP, int INF>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE, INF>()))))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_waves<T, MODE, INF, XP>()))))
     fft_x_kernel(SPME_FRAME_ARGS, int sign, int finish) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int EPT = kXElems / kBlock;
@@ -739,15 +872,21 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
         load_twiddles<T>(twl, F.tw0, N);
         if (MODE != 0)
             for (int e = threadIdx.x; e < N; e += kBlock) ib0[e] = F.iB0[e];
-        if (MODE == 2 && !INF && F.tabmode == kTableAnalytic && F.tcell.iL10 == 0.0 &&
-            F.tcell.iL20 == 0.0) {
-            // gx[kx] = e^(fac iL00^2 nx^2), after the lines and the wavefront sums
-            double *gx = reinterpret_cast<double *>(smem_raw + xlines_lds_bytes<T>(N, C, XP == kXGeneric) +
-                                                    kXWaveSums);
+        if (MODE == 2 && INF != 1 && F.tabmode == kTableAnalytic && x_separable(F.tcell)) {
+            // gx[|nx|] = e^(fac iL00^2 nx^2) and the column constants, after the
+            // lines and the wavefront sums
+            double *gx = reinterpret_cast<double *>(
+                smem_raw + xlines_lds_bytes<T>(N, C, XP == kXGeneric) + kXWaveSums);
             const double q00 = F.tcell.iL00 * F.tcell.iL00;
-            for (int e = threadIdx.x; e < N; e += kBlock) {
-                const double nx = e <= N / 2 ? e : e - N;
+            for (int e = threadIdx.x; e <= N / 2; e += kBlock) {
+                const double nx = e;
                 gx[e] = exp_nonpos(F.ewald_fac * q00 * nx * nx);
+            }
+            if ((int)threadIdx.x < C) {
+                const int fk = min(b * C + (int)threadIdx.x, P - 1);
+                const int ky = fdiv(fk, F.dldim);
+                xsep_column(F, ky, fk - ky * ldim,
+                            gx + xfused_gx_doubles(N) + kXColConsts * threadIdx.x);
             }
         }
 #pragma unroll
@@ -773,9 +912,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
         const int ky = fdiv(fv, F.dldim), kz = fv - ky * ldim;
         double *wsums = reinterpret_cast<double *>(
             smem_raw + xlines_lds_bytes<T>(N, C, XP == kXGeneric));
-        if (!INF && F.tabmode == kTableAnalytic) {
-            x_influence64<T>(F, res, ib0, wsums + 8 * (kBlock / 64), LS, c, e0, es, valid, ky,
-                             kz, F.iB1[ky] * F.iB2[kz], wsums);
+        const double *gx = wsums + 8 * (kBlock / 64);
+        const double *colk = gx + xfused_gx_doubles(N);
+        if constexpr (INF == 2) {
+            x_influence64_xsep<T>(F, res, ib0, gx, colk, LS, c, e0, es, valid, ky, kz, wsums);
+        } else if (!INF && F.tabmode == kTableAnalytic) {
+            if (x_separable(F.tcell))
+                x_influence64_xsep<T>(F, res, ib0, gx, colk, LS, c, e0, es, valid, ky, kz,
+                                      wsums);
+            else
+                x_influence64_gen<T>(F, res, ib0, LS, c, e0, es, valid, ky, kz,
+                                     F.iB1[ky] * F.iB2[kz], wsums);
         } else {
             double acc[7];
             x_influence<T, INF>(F, res, ib0, LS, c, e0, es, f, valid, ky, kz,

@@ -23,6 +23,7 @@
 // fraction f adds w M(f0+i) M(f1+j) M(f2+k) to ((c + n/2 - i) mod K, ...).
 #pragma once
 
+#include <cstddef>
 #include <type_traits>
 
 #include "common.hpp"
@@ -113,6 +114,29 @@ __device__ __forceinline__ AtomRec<T> make_rec(const FrameDesc &F, int a, int &c
     return r;
 }
 
+// the same record as raw 16-byte words (the layout of AtomRec<T>)
+template <typename T>
+__device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx, uint4 *v) {
+    int c[3];
+    double f[3];
+    atom_cell(F, a, c, f);
+    cx = c[0];
+    const unsigned cell = (unsigned)((c[0] << 20) | (c[1] << 10) | c[2]);
+    const unsigned long long wb = (unsigned long long)__double_as_longlong(F.w[a]);
+    if constexpr (std::is_same<T, float>::value) {
+        v[0] = make_uint4(__float_as_uint((float)f[0]), __float_as_uint((float)f[1]),
+                          __float_as_uint((float)f[2]), cell);
+        v[1] = make_uint4((unsigned)wb, (unsigned)(wb >> 32), (unsigned)a, 0u);
+    } else {
+        const unsigned long long f0 = (unsigned long long)__double_as_longlong(f[0]);
+        const unsigned long long f1 = (unsigned long long)__double_as_longlong(f[1]);
+        const unsigned long long f2 = (unsigned long long)__double_as_longlong(f[2]);
+        v[0] = make_uint4((unsigned)f0, (unsigned)(f0 >> 32), (unsigned)f1, (unsigned)(f1 >> 32));
+        v[1] = make_uint4((unsigned)f2, (unsigned)(f2 >> 32), (unsigned)wb, (unsigned)(wb >> 32));
+        v[2] = make_uint4(cell, (unsigned)a, 0u, 0u);
+    }
+}
+
 // ---- bin: atoms into x-plane buckets ------------------------------------------------
 // kBinAtoms atoms per workgroup (kBinAtoms / kBinThreads per thread, all loads
 // in flight together).  F.planecount[0..K0) counts each plane's atoms over the
@@ -182,6 +206,17 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
 // through per-column LDS cursors.  The order within a column is the order of
 // the cursor atomics; no result depends on it (fixed-point sums are exact
 // integers and every atom's gather is its own).
+// the packed cell of a record held as raw 16-byte words
+template <typename T> __device__ __forceinline__ int raw_cell(const uint4 *v);
+template <> __device__ __forceinline__ int raw_cell<float>(const uint4 *v) {
+    static_assert(offsetof(AtomRec<float>, cell) == 12, "cell is word 3 of the first quad");
+    return (int)v[0].w;
+}
+template <> __device__ __forceinline__ int raw_cell<double>(const uint4 *v) {
+    static_assert(offsetof(AtomRec<double>, cell) == 32, "cell is word 0 of the third quad");
+    return (int)v[2].x;
+}
+
 constexpr int kSortThreads = 512;
 constexpr int kSortRegs = 4;  // records per thread held in registers (staged sort)
 
@@ -210,12 +245,17 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     // staged: the whole plane from its bucket into registers (one read), its
     // records placed in column order in LDS, then written out contiguously
     const bool staged = novf == 0 && sort_lds_bytes<T>(F.bcap) > 0;
-    AtomRec<T> r[kSortRegs];
+    // the staged records as raw 16-byte words (a register array of the record
+    // structs themselves was placed in scratch memory)
+    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    uint4 r[kSortRegs][RW];
     if (staged) {
+        const uint4 *bk4 = reinterpret_cast<const uint4 *>(bk);
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q) {
             const int i = q * kSortThreads + (int)threadIdx.x;
-            if (i < nb) r[q] = bk[i];
+#pragma unroll
+            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bk4[(long)i * RW + w] : make_uint4(0, 0, 0, 0);
         }
     }
     // atoms in the planes before this one
@@ -231,14 +271,15 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     if (staged) {
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
-            if (q * kSortThreads + (int)threadIdx.x < nb) atomicAdd(&cur[(r[q].cell >> 10) & 1023], 1);
+            if (q * kSortThreads + (int)threadIdx.x < nb) atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
     } else {
         for (int i = threadIdx.x; i < nb; i += blockDim.x)
             atomicAdd(&cur[(bk[i].cell >> 10) & 1023], 1);
         for (int i = threadIdx.x; i < novf; i += blockDim.x) {
             int c;
-            const AtomRec<T> o = make_rec<T>(F, F.ovf[i], c);
-            if (c == cx) atomicAdd(&cur[(o.cell >> 10) & 1023], 1);
+            uint4 v[RW];
+            make_rec_raw<T>(F, F.ovf[i], c, v);
+            if (c == cx) atomicAdd(&cur[(raw_cell<T>(v) >> 10) & 1023], 1);
         }
     }
     __syncthreads();
@@ -273,26 +314,39 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     __syncthreads();
     AtomRec<T> *recs = (AtomRec<T> *)F.recs + pre;
     if (staged) {
-        AtomRec<T> *sbuf = reinterpret_cast<AtomRec<T> *>(smem_raw);
+        uint4 *sbuf = reinterpret_cast<uint4 *>(smem_raw);
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
-            if (q * kSortThreads + (int)threadIdx.x < nb)
-                sbuf[atomicAdd(&cur[(r[q].cell >> 10) & 1023], 1)] = r[q];
+            if (q * kSortThreads + (int)threadIdx.x < nb) {
+                const int slot = atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) sbuf[slot * RW + w] = r[q][w];
+            }
         __syncthreads();
         // contiguous write-out, 16 bytes per lane
-        const uint4 *src = reinterpret_cast<const uint4 *>(sbuf);
         uint4 *dst = reinterpret_cast<uint4 *>(recs);
-        constexpr int W = sizeof(AtomRec<T>) / sizeof(uint4);
-        for (int i = threadIdx.x; i < nb * W; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
     } else {
+        const uint4 *bk4 = reinterpret_cast<const uint4 *>(bk);
+        uint4 *rec4 = reinterpret_cast<uint4 *>(recs);
         for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-            const AtomRec<T> o = bk[i];
-            recs[atomicAdd(&cur[(o.cell >> 10) & 1023], 1)] = o;
+            const uint4 v0 = bk4[(long)i * RW], v1 = bk4[(long)i * RW + 1];
+            const uint4 v2 = RW > 2 ? bk4[(long)i * RW + 2] : v1;
+            const int cell = std::is_same<T, float>::value ? (int)v0.w : (int)v2.x;
+            const int slot = atomicAdd(&cur[(cell >> 10) & 1023], 1);
+            rec4[(long)slot * RW] = v0;
+            rec4[(long)slot * RW + 1] = v1;
+            if (RW > 2) rec4[(long)slot * RW + 2] = v2;
         }
         for (int i = threadIdx.x; i < novf; i += blockDim.x) {
             int c;
-            const AtomRec<T> o = make_rec<T>(F, F.ovf[i], c);
-            if (c == cx) recs[atomicAdd(&cur[(o.cell >> 10) & 1023], 1)] = o;
+            uint4 v[RW];
+            make_rec_raw<T>(F, F.ovf[i], c, v);
+            if (c == cx) {
+                const int slot = atomicAdd(&cur[(raw_cell<T>(v) >> 10) & 1023], 1);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) rec4[(long)slot * RW + w] = v[w];
+            }
         }
     }
     if (cx == 0 && F.fixed) {  // the fixed-point scale, from the bin partial maxima
