@@ -66,7 +66,7 @@ def kernel_bytes(kernel, n, K, p):
     G = K0 * K1 * K2
     H = K0 * K1 * (K2 // 2 + 1)
     table = {
-        "bin": n * 4 * 8 + n * (3 * p + 16),   # x, w in; records out (plane buckets)
+        "bin": n * 4 * 8 + n * (3 * p + 16),   # x, w in; records out (plane-ordered chunks)
         "sort": 2 * n * (3 * p + 16),          # records in, records out in column order
         "spread": n * (3 * p + 16) + G * p,    # column-sorted records in, grid out once
         "fft_z_r2c": G * p + 2 * H * p,

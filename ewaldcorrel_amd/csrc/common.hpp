@@ -141,11 +141,10 @@ struct FrameDesc {
     double *dx, *phi;
     double *out;  // E, virial[6]
     // sort + spreading workspace
-    int *planecount;  // K0 plane counters and the overflow count, zero between evaluations
     int *colstart;    // K0*K1 + 1 exclusive prefix
-    void *bucket;     // AtomRec<T>[K0][bcap]: each plane's records in bin order
-    int bcap;         // records a plane's bucket holds
-    int *ovf;         // indices of the atoms beyond their plane's bucket
+    void *bins;       // AtomRec<T>[n]: each bin workgroup's chunk in x-plane order
+    int *binstart;    // [bin][K0 + 1]: plane starts within each chunk
+    int scap;         // records of a plane the sort stages in LDS
     void *recs;       // AtomRec<T>[n] in column order
     double *wmax;     // max |w| of each bin workgroup (fixed point), reduced by the sort
     double *fxscale;  // fixed-point scale, written by the sort

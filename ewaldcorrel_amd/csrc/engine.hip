@@ -151,7 +151,9 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
               hipStream_t st, const SplineArgs<T> &sp, bool fixed, int finish) {
     const dim3 g(blocks), bl(kBlock);
     switch (s) {
-        case ST_BIN: bin_kernel<T><<<g, kBinThreads, 0, st>>>(L.frames, L.offs, L.nf); break;
+        case ST_BIN:
+            bin_kernel<T><<<g, kBinThreads, bin_lds_bytes<T>(), st>>>(L.frames, L.offs, L.nf);
+            break;
         case ST_SORT: sort_kernel<T><<<g, kSortThreads, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_SPREAD: {
             const dim3 sb(kSpreadBlock);
@@ -549,11 +551,9 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     HIPC(hipGetDevice(&dev_));
     memset(&desc_uploaded_, 0xff, sizeof(FrameDesc));
     alloc_((void **)&D.partials, (size_t)nred_max * 8 * sizeof(double));
-    alloc_((void **)&D.planecount, ((size_t)K[0] + 1) * sizeof(int));
     alloc_((void **)&D.colstart, ((size_t)nrows + 1) * sizeof(int));
     alloc_((void **)&D.fxscale, 64);
     alloc_((void **)&result_, 8 * sizeof(double));
-    HIPC(hipMemsetAsync(D.planecount, 0, ((size_t)K[0] + 1) * sizeof(int), stream_));
     HIPC(hipMemsetAsync(grid_, 0, grid_reals() * tsz, stream_));
     HIPC(hipMemsetAsync(result_, 0, 8 * sizeof(double), stream_));
     D.out = result_;
@@ -740,33 +740,40 @@ void Engine::ensure_atoms(int n) {
     }
 }
 
-// The per-atom sort workspace of n atoms in one block: the plane buckets
-// (K0 of bucket_cap(n) records: 1.5 times a uniform plane's share plus 64, so
-// that an even distribution never overflows), the overflow list (n indices),
-// the column-ordered records and one |w| maximum per bin workgroup, each part
-// 256-byte aligned.  Nothing in it needs to start at zero.
+// The per-atom sort workspace of n atoms in one block: the bin chunks (n
+// records in x-plane order per bin workgroup), the plane starts of each chunk
+// (K0 + 1 ints per bin workgroup), the column-ordered records and one |w|
+// maximum per bin workgroup, each part 256-byte aligned.  Nothing in it needs
+// to start at zero.
 namespace {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
-int Engine::bucket_cap(size_t n) const {
-    return (int)((3 * n + 2 * (size_t)K[0] - 1) / (2 * (size_t)K[0]) + 64);
+// Records of one plane the sort stages in LDS: 1.5 times a uniform plane's
+// share plus 64 (a plane beyond it is sorted straight into place), within
+// 64 KB.
+int Engine::sort_cap(size_t n) const {
+    const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
+    const size_t want = (3 * n + 2 * (size_t)K[0] - 1) / (2 * (size_t)K[0]) + 64;
+    return (int)std::min(want, (size_t)65536 / rec);
 }
 
 size_t Engine::workspace_bytes(size_t n) const {
     const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
-    return align256((size_t)K[0] * bucket_cap(n) * rec) + align256(n * sizeof(int)) +
-           align256(n * rec) + align256((n + kBinAtoms - 1) / kBinAtoms * sizeof(double));
+    const size_t nbin = (size_t)bin_count((int)n, fp32);
+    return align256(n * rec) + align256(nbin * ((size_t)K[0] + 1) * sizeof(int)) +
+           align256(n * rec) + align256(nbin * sizeof(double));
 }
 
 void Engine::bind_workspace(FrameDesc &D, void *base, size_t n) const {
     const size_t rec = fp32 ? sizeof(AtomRec<float>) : sizeof(AtomRec<double>);
+    const size_t nbin = (size_t)bin_count((int)n, fp32);
     char *p = static_cast<char *>(base);
-    D.bcap = bucket_cap(n);
-    D.bucket = p;
-    p += align256((size_t)K[0] * D.bcap * rec);
-    D.ovf = reinterpret_cast<int *>(p);
-    p += align256(n * sizeof(int));
+    D.scap = sort_cap(n);
+    D.bins = p;
+    p += align256(n * rec);
+    D.binstart = reinterpret_cast<int *>(p);
+    p += align256(nbin * ((size_t)K[0] + 1) * sizeof(int));
     D.recs = p;
     p += align256(n * rec);
     D.wmax = reinterpret_cast<double *>(p);
@@ -802,7 +809,7 @@ int Engine::stage_blocks_n(Stage s, int natoms) const {
     const FrameDesc &D = desc_;
     const int nrows = K[0] * K[1];
     switch (s) {
-        case ST_BIN: return (natoms + kBinAtoms - 1) / kBinAtoms;
+        case ST_BIN: return bin_count(natoms, fp32);
         case ST_SORT: return K[0];
         case ST_SPREAD: return D.ntx * D.nty * D.ntz;
         case ST_ZR2C:
@@ -834,7 +841,7 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
     const int xgen = std::max(xpath, x_path(D)) == kXGeneric;
     switch (s) {
         case ST_SORT:  // the staged plane
-            return fp32 ? sort_lds_bytes<float>(D.bcap) : sort_lds_bytes<double>(D.bcap);
+            return fp32 ? sort_lds_bytes<float>(D.scap) : sort_lds_bytes<double>(D.scap);
         case ST_SPREAD:  // chunked tiles: the compacted atom list after the cells
             return (size_t)D.tx * D.ty * (D.ntz > 1 ? D.tz : K[2] + order - 1) * 8 +
                    (D.ntz > 1 ? kSpreadBlock * sizeof(int) : 0);
@@ -1031,10 +1038,10 @@ void Engine::evaluate_noalloc(int n, const double *w_d, const double *x_d, doubl
     Stage prev = ST_COUNT;
     for (int i = 0; i < seq.n; ++i) {
         const Stage st = seq.s[i];
-        // the sort's LDS follows the bucket capacity of this call's workspace
+        // the sort's LDS follows the staging capacity of this call's workspace
         const size_t lds = st != ST_SORT ? stage_lds(st)
-                           : fp32        ? sort_lds_bytes<float>(desc_na_.bcap)
-                                         : sort_lds_bytes<double>(desc_na_.bcap);
+                           : fp32        ? sort_lds_bytes<float>(desc_na_.scap)
+                                         : sort_lds_bytes<double>(desc_na_.scap);
         launch(st, L, stage_blocks_n(st, n), lds, stream_, finishes(prev, st));
         prev = st;
     }

@@ -225,7 +225,7 @@ public:
     bool uses_stage(Stage s) const;
     // per-atom sort workspace: bytes for n atoms, and its parts in D at base
     size_t workspace_bytes(size_t n) const;
-    int bucket_cap(size_t n) const;  // records per plane bucket for n atoms
+    int sort_cap(size_t n) const;  // records of a plane the sort stages in LDS, n atoms
     void bind_workspace(FrameDesc &D, void *base, size_t n) const;
     // xpath: the x path of the launch (kXBlock, or kXGeneric for a batch with
     // a generic frame); the frame's own path when that is more general

@@ -3,22 +3,19 @@
 // The reference spreads in gather form, one grid point at a time, with
 // order^3 multimap lookups per point (sfac.cpp:113-124,141-168).  Here:
 //   bin      per atom: cell c = (int)(s K) and its record (fraction, w, cell,
-//            original index) into the bucket of its x plane cx; a workgroup
-//            counts its atoms per plane in LDS and reserves each plane's slots
-//            with one global atomic (not one per atom); atoms beyond a
-//            bucket's capacity go to an overflow list (by index); block max |w|
-//   sort     per (frame, x plane): counting sort of the plane's records by
-//            column cy in LDS; writes the column starts and the records in
-//            column order (frame-wide: planes in x order, columns in y
-//            order); plane 0 also fixes the fixed-point scale
+//            original index); each workgroup counting-sorts its own atoms by
+//            x plane in LDS and writes them as one contiguous chunk with the
+//            plane starts (no global atomics, no capacity limits); block max |w|
+//   sort     per (frame, x plane): counting sort of the plane's runs in all
+//            chunks by column cy in LDS; writes the column starts and the
+//            records in column order (frame-wide: planes in x order, columns
+//            in y order); plane 0 also fixes the fixed-point scale
 //   spread   per x-y tile of full-z columns: every atom of the
 //            (tile + order - 1) neighbourhood adds the part of its stencil
 //            that falls in the tile to an LDS accumulator (int64 fixed
 //            point or FP64 LDS atomics); the tile is then written once with
 //            coalesced stores.  No global atomics, no separate zeroing or
 //            conversion pass, and every grid point is written exactly once.
-//            The first tile of each frame clears the plane counters for the
-//            next call.
 // The stencil convention is the reference's: an atom in cell c with
 // fraction f adds w M(f0+i) M(f1+j) M(f2+k) to ((c + n/2 - i) mod K, ...).
 #pragma once
@@ -97,24 +94,8 @@ __device__ __forceinline__ double fixed_scale(int n, double wmax, double mmax3, 
     return ldexp(1.0, (fx32 ? 30 : 62) - e);
 }
 
-// the record of atom a (scale_wrap, cell and fraction as atom_cell)
-template <typename T>
-__device__ __forceinline__ AtomRec<T> make_rec(const FrameDesc &F, int a, int &cx) {
-    int c[3];
-    double f[3];
-    atom_cell(F, a, c, f);
-    AtomRec<T> r;
-    r.f[0] = T(f[0]);
-    r.f[1] = T(f[1]);
-    r.f[2] = T(f[2]);
-    r.w = F.w[a];
-    r.cell = (c[0] << 20) | (c[1] << 10) | c[2];
-    r.perm = a;
-    cx = c[0];
-    return r;
-}
-
-// the same record as raw 16-byte words (the layout of AtomRec<T>)
+// the record of atom a (scale_wrap, cell and fraction as atom_cell) as raw
+// 16-byte words in the layout of AtomRec<T>
 template <typename T>
 __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx, uint4 *v) {
     int c[3];
@@ -137,55 +118,100 @@ __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx,
     }
 }
 
-// ---- bin: atoms into x-plane buckets ------------------------------------------------
-// kBinAtoms atoms per workgroup (kBinAtoms / kBinThreads per thread, all loads
-// in flight together).  F.planecount[0..K0) counts each plane's atoms over the
-// whole frame (the bucket holds the first F.bcap of them), F.planecount[K0] the
-// overflow list's length; both are zero when the launch starts.
+// ---- bin: each workgroup's atoms in x-plane order ------------------------------------------
+// Workgroup b takes atoms [b BA, b BA + BA) (BA = bin_atoms<T>(), all loads in
+// flight together), computes their records, counts them per x plane in LDS,
+// places them in plane order in LDS and writes them as one contiguous chunk
+// bins[b BA ..); binstart[b][p] (p <= K0) is the start of plane p within the
+// chunk (binstart[b][K0] = the chunk's atoms).  No global atomics and no
+// capacity limit: a plane may hold any number of atoms.
 constexpr int kBinThreads = 1024;
-constexpr int kBinAtoms = 2048;
+template <typename T> __host__ __device__ constexpr int bin_atoms() {
+    return std::is_same<T, float>::value ? 2048 : 1024;  // 64 KB / 48 KB of records in LDS
+}
+template <typename T> __host__ __device__ constexpr size_t bin_lds_bytes() {
+    return (size_t)bin_atoms<T>() * sizeof(AtomRec<T>);
+}
+__host__ __device__ inline int bin_count(int n, bool fp32) {
+    const int ba = fp32 ? bin_atoms<float>() : bin_atoms<double>();
+    return (n + ba - 1) / ba;
+}
+
+// exclusive prefix of v over the workgroup (blockDim.x threads, <= 1024, in
+// thread order); *total receives the sum.  wt: blockDim.x / 64 ints of LDS.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *wt, int *total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wt[wv] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int x = wt[k];
+        off += k < wv ? x : 0;
+        tot += x;
+    }
+    __syncthreads();  // wt may be reused once every thread has read it
+    *total = tot;
+    return off + incl - v;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
-    constexpr int A = kBinAtoms / kBinThreads;
-    __shared__ int pl[kMaxK];  // per-plane count, then this workgroup's base in the plane
+    constexpr int BA = bin_atoms<T>(), A = BA / kBinThreads;
+    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    extern __shared__ __align__(16) unsigned char smem_raw[];  // BA records in plane order
+    __shared__ int pl[kMaxK];  // per-plane count, then the plane's start in the chunk
+    __shared__ int wt[kBinThreads / 64];
     __shared__ double red[kBinThreads / 64];
     int b;
     const FrameDesc &F = frames[find_frame(offs, nf, b)];
     const int K0 = F.K0;
     for (int p = threadIdx.x; p < K0; p += blockDim.x) pl[p] = 0;
     __syncthreads();
-    AtomRec<T> r[A];
+    uint4 r[A][RW];
     int cx[A], lr[A];
     double wm = 0.0;
 #pragma unroll
     for (int q = 0; q < A; ++q) {
-        const int a = b * kBinAtoms + q * kBinThreads + (int)threadIdx.x;
+        const int a = b * BA + q * kBinThreads + (int)threadIdx.x;
         cx[q] = -1;
         if (a < F.n) {
-            r[q] = make_rec<T>(F, a, cx[q]);
-            wm = fmax(wm, fabs(r[q].w));
+            make_rec_raw<T>(F, a, cx[q], r[q]);
+            wm = fmax(wm, fabs(F.w[a]));
             lr[q] = atomicAdd(&pl[cx[q]], 1);
         }
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < K0; p += blockDim.x) {
-        const int h = pl[p];
-        if (h) pl[p] = atomicAdd(F.planecount + p, h);
+    // plane starts (K0 <= 1024 = kBinThreads: one plane per thread)
+    {
+        const int p = threadIdx.x;
+        int tot;
+        const int st = block_exclusive_scan(p < K0 ? pl[p] : 0, wt, &tot);
+        int *bs = F.binstart + (long)b * (K0 + 1);
+        if (p < K0) {
+            pl[p] = st;
+            bs[p] = st;
+        }
+        if (p == 0) bs[K0] = tot;
     }
     __syncthreads();
-    AtomRec<T> *bucket = (AtomRec<T> *)F.bucket;
+    uint4 *stage = reinterpret_cast<uint4 *>(smem_raw);
 #pragma unroll
     for (int q = 0; q < A; ++q) {
         if (cx[q] < 0) continue;
         const int slot = pl[cx[q]] + lr[q];
-        if (slot < F.bcap) {
-            bucket[(long)cx[q] * F.bcap + slot] = r[q];
-        } else {  // a crowded plane: the atom goes to the overflow list
-            const int o = atomicAdd(F.planecount + K0, 1);
-            F.ovf[o] = r[q].perm;
-        }
+#pragma unroll
+        for (int w = 0; w < RW; ++w) stage[slot * RW + w] = r[q][w];
     }
+    __syncthreads();
+    const int nb = min(BA, F.n - b * BA);
+    uint4 *dst = reinterpret_cast<uint4 *>(F.bins) + (long)b * BA * RW;
+    for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = stage[i];
     if (F.fixed) {  // one partial maximum per workgroup
         for (int off = 32; off > 0; off >>= 1) wm = fmax(wm, __shfl_xor(wm, off));
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -200,12 +226,16 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
 }
 
 // ---- sort: one workgroup per (frame, x plane) ------------------------------------
-// Counting sort of the plane's bucket (and of its atoms on the overflow list)
-// by column: counts per column in LDS, an exclusive scan, the column starts
-// (offset by the atoms of all earlier planes), then every record to its place
-// through per-column LDS cursors.  The order within a column is the order of
-// the cursor atomics; no result depends on it (fixed-point sums are exact
-// integers and every atom's gather is its own).
+// Counting sort of the plane's records by column: the plane's run in every
+// bin chunk is read by one wavefront (lanes along the run), once to count
+// the columns and once to place the records; column starts (offset by the
+// atoms of all earlier planes) written; the records go through LDS and out
+// contiguously when the plane fits F.scap records, else straight to their
+// place.  The order within a column is the order of the cursor atomics; no
+// result depends on it (fixed-point sums are exact integers and every
+// atom's gather is its own).
+constexpr int kSortThreads = 512;
+
 // the packed cell of a record held as raw 16-byte words
 template <typename T> __device__ __forceinline__ int raw_cell(const uint4 *v);
 template <> __device__ __forceinline__ int raw_cell<float>(const uint4 *v) {
@@ -216,71 +246,64 @@ template <> __device__ __forceinline__ int raw_cell<double>(const uint4 *v) {
     static_assert(offsetof(AtomRec<double>, cell) == 32, "cell is word 0 of the third quad");
     return (int)v[2].x;
 }
+// the packed cell of record i of a chunk, loaded alone
+template <typename T> __device__ __forceinline__ int rec_cell(const uint4 *chunk, int i) {
+    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    typedef __attribute__((address_space(1))) const int GI;
+    GI *wd = (GI *)(chunk + (long)i * RW);
+    return std::is_same<T, float>::value ? wd[3] : wd[8];
+}
 
-constexpr int kSortThreads = 512;
-constexpr int kSortRegs = 4;  // records per thread held in registers (staged sort)
-
-// LDS of the staged sort: a plane's bucket in column order (0: the planes are
-// sorted in place, directly from the bucket to the output)
-template <typename T> __host__ __device__ inline size_t sort_lds_bytes(int bcap) {
-    const size_t b = (size_t)bcap * sizeof(AtomRec<T>);
-    return bcap <= kSortThreads * kSortRegs && b <= 64 * 1024 ? b : 0;
+// records of the staged sort that fit in LDS (the launch gives scap records)
+template <typename T> __host__ __device__ inline size_t sort_lds_bytes(int scap) {
+    return (size_t)scap * sizeof(AtomRec<T>);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
+    constexpr int BA = bin_atoms<T>();
+    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int cur[kMaxK];  // column counts, then their starts, then cursors
     __shared__ int wtot[kSortThreads / 64];
+    __shared__ int wcnt[kSortThreads / 64];
     __shared__ double wred[kSortThreads / 64];
     int cx;
     const FrameDesc &F = frames[find_frame(offs, nf, cx)];
-    const int K0 = F.K0, K1 = F.K1;
+    const int K0 = F.K0, K1 = F.K1, S = K0 + 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     constexpr int NW = kSortThreads / 64;
-    const int cnt = F.planecount[cx];
-    const int nb = min(cnt, F.bcap);
-    const int novf = F.planecount[K0];
-    const AtomRec<T> *bk = (const AtomRec<T> *)F.bucket + (long)cx * F.bcap;
-    // staged: the whole plane from its bucket into registers (one read), its
-    // records placed in column order in LDS, then written out contiguously
-    const bool staged = novf == 0 && sort_lds_bytes<T>(F.bcap) > 0;
-    // the staged records as raw 16-byte words (a register array of the record
-    // structs themselves was placed in scratch memory)
-    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
-    uint4 r[kSortRegs][RW];
-    if (staged) {
-        const uint4 *bk4 = reinterpret_cast<const uint4 *>(bk);
-#pragma unroll
-        for (int q = 0; q < kSortRegs; ++q) {
-            const int i = q * kSortThreads + (int)threadIdx.x;
-#pragma unroll
-            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bk4[(long)i * RW + w] : make_uint4(0, 0, 0, 0);
-        }
+    const int nbin = (F.n + BA - 1) / BA;
+    const int *bs = F.binstart;
+    const uint4 *bins = reinterpret_cast<const uint4 *>(F.bins);
+    // atoms of this plane and of the planes before it
+    int part = 0, cnt = 0;
+    for (int k = threadIdx.x; k < nbin; k += blockDim.x) {
+        const int s0 = bs[(long)k * S + cx];
+        part += s0;
+        cnt += bs[(long)k * S + cx + 1] - s0;
     }
-    // atoms in the planes before this one
-    int part = 0;
-    for (int p = threadIdx.x; p < cx; p += blockDim.x) part += F.planecount[p];
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-    if (lane == 0) wtot[wv] = part;
+    for (int off = 32; off > 0; off >>= 1) {
+        part += __shfl_xor(part, off);
+        cnt += __shfl_xor(cnt, off);
+    }
+    if (lane == 0) {
+        wtot[wv] = part;
+        wcnt[wv] = cnt;
+    }
     for (int y = threadIdx.x; y < K1; y += blockDim.x) cur[y] = 0;
     __syncthreads();
-    int pre = 0;
+    int pre = 0, nb = 0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) pre += wtot[k];
-    if (staged) {
-#pragma unroll
-        for (int q = 0; q < kSortRegs; ++q)
-            if (q * kSortThreads + (int)threadIdx.x < nb) atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
-    } else {
-        for (int i = threadIdx.x; i < nb; i += blockDim.x)
-            atomicAdd(&cur[(bk[i].cell >> 10) & 1023], 1);
-        for (int i = threadIdx.x; i < novf; i += blockDim.x) {
-            int c;
-            uint4 v[RW];
-            make_rec_raw<T>(F, F.ovf[i], c, v);
-            if (c == cx) atomicAdd(&cur[(raw_cell<T>(v) >> 10) & 1023], 1);
-        }
+    for (int k = 0; k < NW; ++k) {
+        pre += wtot[k];
+        nb += wcnt[k];
+    }
+    // pass 1: column counts (wavefront k takes the runs of chunks k, k + NW, ...)
+    for (int k = wv; k < nbin; k += NW) {
+        const int s0 = bs[(long)k * S + cx], c = bs[(long)k * S + cx + 1] - s0;
+        const uint4 *run = bins + ((long)k * BA + s0) * RW;
+        for (int j = lane; j < c; j += 64) atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
     }
     __syncthreads();
     // exclusive scan of cur[0..K1): two consecutive columns per thread
@@ -309,48 +332,41 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             cur[y0 + 1] = e0 + v0;
             F.colstart[cx * K1 + y0 + 1] = pre + e0 + v0;
         }
-        if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + cnt;
+        if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + nb;
     }
     __syncthreads();
-    AtomRec<T> *recs = (AtomRec<T> *)F.recs + pre;
-    if (staged) {
-        uint4 *sbuf = reinterpret_cast<uint4 *>(smem_raw);
-#pragma unroll
-        for (int q = 0; q < kSortRegs; ++q)
-            if (q * kSortThreads + (int)threadIdx.x < nb) {
-                const int slot = atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
-#pragma unroll
-                for (int w = 0; w < RW; ++w) sbuf[slot * RW + w] = r[q][w];
+    // pass 2: every record to its place, through LDS when the plane fits
+    typedef __attribute__((address_space(1))) uint4 GQ;
+    typedef __attribute__((address_space(3))) uint4 LQ;
+    GQ *dst = (GQ *)F.recs + (long)pre * RW;
+    LQ *sbuf = (LQ *)smem_raw;
+    // (separate loops for the two destinations: with one pointer of either
+    // kind the compiler kept the loaded record in scratch memory)
+    auto place_all = [&](auto *where) {
+        for (int k = wv; k < nbin; k += NW) {
+            const int s0 = bs[(long)k * S + cx], c = bs[(long)k * S + cx + 1] - s0;
+            const GQ *run = (const GQ *)bins + ((long)k * BA + s0) * RW;
+            for (int j = lane; j < c; j += 64) {
+                const uint4 v0 = run[(long)j * RW], v1 = run[(long)j * RW + 1];
+                const uint4 v2 = RW > 2 ? run[(long)j * RW + 2] : v1;
+                // the cell by a load of its own: taken from the loaded words,
+                // the compiler moved the rest of them through scratch memory
+                const int cell = rec_cell<T>((const uint4 *)run, j);
+                const long slot = atomicAdd(&cur[(cell >> 10) & 1023], 1);
+                where[slot * RW] = v0;
+                where[slot * RW + 1] = v1;
+                if (RW > 2) where[slot * RW + 2] = v2;
             }
-        __syncthreads();
-        // contiguous write-out, 16 bytes per lane
-        uint4 *dst = reinterpret_cast<uint4 *>(recs);
+        }
+    };
+    if (nb <= F.scap) {
+        place_all(sbuf);
+        __syncthreads();  // contiguous write-out, 16 bytes per lane
         for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
     } else {
-        const uint4 *bk4 = reinterpret_cast<const uint4 *>(bk);
-        uint4 *rec4 = reinterpret_cast<uint4 *>(recs);
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-            const uint4 v0 = bk4[(long)i * RW], v1 = bk4[(long)i * RW + 1];
-            const uint4 v2 = RW > 2 ? bk4[(long)i * RW + 2] : v1;
-            const int cell = std::is_same<T, float>::value ? (int)v0.w : (int)v2.x;
-            const int slot = atomicAdd(&cur[(cell >> 10) & 1023], 1);
-            rec4[(long)slot * RW] = v0;
-            rec4[(long)slot * RW + 1] = v1;
-            if (RW > 2) rec4[(long)slot * RW + 2] = v2;
-        }
-        for (int i = threadIdx.x; i < novf; i += blockDim.x) {
-            int c;
-            uint4 v[RW];
-            make_rec_raw<T>(F, F.ovf[i], c, v);
-            if (c == cx) {
-                const int slot = atomicAdd(&cur[(raw_cell<T>(v) >> 10) & 1023], 1);
-#pragma unroll
-                for (int w = 0; w < RW; ++w) rec4[(long)slot * RW + w] = v[w];
-            }
-        }
+        place_all(dst);
     }
     if (cx == 0 && F.fixed) {  // the fixed-point scale, from the bin partial maxima
-        const int nbin = (F.n + kBinAtoms - 1) / kBinAtoms;
         double m = 0.0;
         for (int i = threadIdx.x; i < nbin; i += blockDim.x) m = fmax(m, F.wmax[i]);
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
@@ -407,8 +423,6 @@ spread_tile_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
 
     typedef typename std::conditional<FIXED, unsigned long long, double>::type Acc;
     Acc *acc = reinterpret_cast<Acc *>(smem_raw);
-    if (b == 0)  // the sort has consumed the plane counters: cleared for the next call
-        for (int p = threadIdx.x; p <= K0; p += blockDim.x) F.planecount[p] = 0;
     {
         uint4 *z4 = reinterpret_cast<uint4 *>(smem_raw);
         for (int i = threadIdx.x; i < cells / 2; i += blockDim.x) z4[i] = make_uint4(0, 0, 0, 0);

@@ -1,6 +1,7 @@
 """The spreading sort (x-plane binning, per-plane column counting sort) on
-inputs that stress it: planes more crowded than their bucket (the overflow
-list), every atom in one column, most planes empty, one atom.  Each case
+inputs that stress it: planes more crowded than the sort stages in LDS
+(sorted straight into place), every atom in one column, most planes empty,
+one atom.  Each case
 against the FP64 oracle, and in the deterministic modes bitwise invariant
 under a permutation of the atoms (the order within a column is the order
 of the sort's atomics, which no result may depend on)."""
