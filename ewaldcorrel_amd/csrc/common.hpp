@@ -143,7 +143,7 @@ struct FrameDesc {
     // sort + spreading workspace
     int *colstart;    // K0*K1 + 1 exclusive prefix
     void *bins;       // AtomRec<T>[n]: each bin workgroup's chunk in x-plane order
-    int *binstart;    // [bin][K0 + 1]: plane starts within each chunk
+    int *binstart;    // [K0 + 1][bin]: plane starts within each chunk
     int scap;         // records of a plane the sort stages in LDS
     void *recs;       // AtomRec<T>[n] in column order
     double *wmax;     // max |w| of each bin workgroup (fixed point), reduced by the sort

@@ -122,9 +122,9 @@ __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx,
 // Workgroup b takes atoms [b BA, b BA + BA) (BA = bin_atoms<T>(), all loads in
 // flight together), computes their records, counts them per x plane in LDS,
 // places them in plane order in LDS and writes them as one contiguous chunk
-// bins[b BA ..); binstart[b][p] (p <= K0) is the start of plane p within the
-// chunk (binstart[b][K0] = the chunk's atoms).  No global atomics and no
-// capacity limit: a plane may hold any number of atoms.
+// bins[b BA ..); binstart[p][b] (p <= K0, nbin chunks per row) is the start
+// of plane p within chunk b (binstart[K0][b] = the chunk's atoms).  No global
+// atomics and no capacity limit: a plane may hold any number of atoms.
 constexpr int kBinThreads = 1024;
 template <typename T> __host__ __device__ constexpr int bin_atoms() {
     return std::is_same<T, float>::value ? 2048 : 1024;  // 64 KB / 48 KB of records in LDS
@@ -192,12 +192,12 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         const int p = threadIdx.x;
         int tot;
         const int st = block_exclusive_scan(p < K0 ? pl[p] : 0, wt, &tot);
-        int *bs = F.binstart + (long)b * (K0 + 1);
+        const int nbin = (F.n + BA - 1) / BA;
         if (p < K0) {
             pl[p] = st;
-            bs[p] = st;
+            F.binstart[(long)p * nbin + b] = st;
         }
-        if (p == 0) bs[K0] = tot;
+        if (p == 0) F.binstart[(long)K0 * nbin + b] = tot;
     }
     __syncthreads();
     uint4 *stage = reinterpret_cast<uint4 *>(smem_raw);
@@ -226,15 +226,19 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
 }
 
 // ---- sort: one workgroup per (frame, x plane) ------------------------------------
-// Counting sort of the plane's records by column: the plane's run in every
-// bin chunk is read by one wavefront (lanes along the run), once to count
-// the columns and once to place the records; column starts (offset by the
-// atoms of all earlier planes) written; the records go through LDS and out
-// contiguously when the plane fits F.scap records, else straight to their
-// place.  The order within a column is the order of the cursor atomics; no
-// result depends on it (fixed-point sums are exact integers and every
-// atom's gather is its own).
+// Counting sort of the plane's records by column.  The plane's records are
+// its runs in the bin chunks.  Staged (the plane fits F.scap records, at most
+// kSortRegs per thread, at most kSortThreads chunks): a table of the runs in
+// LDS maps record i of the plane to its chunk, each thread loads its records
+// into registers once, counts their columns, and places them in column order
+// in LDS, written out contiguously.  Otherwise each run is read by one
+// wavefront, once to count and once to place the records straight into
+// their place.  Column starts (offset by the atoms of all earlier planes)
+// are written either way.  The order within a column is the order of the
+// cursor atomics; no result depends on it (fixed-point sums are exact
+// integers and every atom's gather is its own).
 constexpr int kSortThreads = 512;
+constexpr int kSortRegs = 4;  // records per thread held in registers (staged sort)
 
 // the packed cell of a record held as raw 16-byte words
 template <typename T> __device__ __forceinline__ int raw_cell(const uint4 *v);
@@ -263,47 +267,84 @@ template <typename T>
 __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     constexpr int BA = bin_atoms<T>();
     constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    constexpr int NT = kSortThreads, NW = kSortThreads / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int cur[kMaxK];  // column counts, then their starts, then cursors
-    __shared__ int wtot[kSortThreads / 64];
-    __shared__ int wcnt[kSortThreads / 64];
-    __shared__ double wred[kSortThreads / 64];
+    __shared__ int rmap[NT * kSortRegs];  // staged: bins index of the plane's record i
+    __shared__ int wtot[NW];
+    __shared__ int wcnt[NW];
+    __shared__ double wred[NW];
     int cx;
     const FrameDesc &F = frames[find_frame(offs, nf, cx)];
-    const int K0 = F.K0, K1 = F.K1, S = K0 + 1;
+    const int K0 = F.K0, K1 = F.K1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    constexpr int NW = kSortThreads / 64;
     const int nbin = (F.n + BA - 1) / BA;
-    const int *bs = F.binstart;
+    // the plane's start and length in every chunk: rows cx and cx + 1
+    const int *row0 = F.binstart + (long)cx * nbin, *row1 = row0 + nbin;
     const uint4 *bins = reinterpret_cast<const uint4 *>(F.bins);
-    // atoms of this plane and of the planes before it
-    int part = 0, cnt = 0;
-    for (int k = threadIdx.x; k < nbin; k += blockDim.x) {
-        const int s0 = bs[(long)k * S + cx];
-        part += s0;
-        cnt += bs[(long)k * S + cx + 1] - s0;
+    // atoms of the planes before this one (part), this plane's run in chunk
+    // k = thread (c0, when there are at most NT chunks) and the runs' wave sums
+    int part = 0, c0 = 0, s00 = 0;
+    if (nbin <= NT) {
+        if ((int)threadIdx.x < nbin) {
+            s00 = row0[threadIdx.x];
+            c0 = row1[threadIdx.x] - s00;
+        }
+        part = s00;
+    } else {
+        for (int k = threadIdx.x; k < nbin; k += blockDim.x) {
+            const int s0 = row0[k];
+            part += s0;
+            c0 += row1[k] - s0;
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        part += __shfl_xor(part, off);
-        cnt += __shfl_xor(cnt, off);
+    int incl = c0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
     }
-    if (lane == 0) {
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 63) {
         wtot[wv] = part;
-        wcnt[wv] = cnt;
+        wcnt[wv] = incl;
     }
     for (int y = threadIdx.x; y < K1; y += blockDim.x) cur[y] = 0;
     __syncthreads();
-    int pre = 0, nb = 0;
+    int pre = 0, nb = 0, woff = 0;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
         pre += wtot[k];
         nb += wcnt[k];
+        woff += k < wv ? wcnt[k] : 0;
     }
-    // pass 1: column counts (wavefront k takes the runs of chunks k, k + NW, ...)
-    for (int k = wv; k < nbin; k += NW) {
-        const int s0 = bs[(long)k * S + cx], c = bs[(long)k * S + cx + 1] - s0;
-        const uint4 *run = bins + ((long)k * BA + s0) * RW;
-        for (int j = lane; j < c; j += 64) atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
+    const bool staged = nb <= F.scap && nb <= NT * kSortRegs && nbin <= NT;
+    uint4 r[kSortRegs][RW];
+    if (staged) {
+        // the map of the plane's records: thread k writes the bins indices of
+        // chunk k's run at the run's place in the plane
+        const int ex = woff + incl - c0, beg = (int)threadIdx.x * BA + s00;
+        for (int j = 0; j < c0; ++j) rmap[ex + j] = beg + j;
+        __syncthreads();
+        // this thread's records into registers (all loads in flight together)
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q) {
+            const int i = q * NT + (int)threadIdx.x;
+            const long g = i < nb ? rmap[i] : 0;
+#pragma unroll
+            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bins[g * RW + w] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q)
+            if (q * NT + (int)threadIdx.x < nb) atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
+    } else {
+        // column counts, a wavefront per run (chunks wv, wv + NW, ...)
+        for (int k = wv; k < nbin; k += NW) {
+            const int s0 = row0[k], c = row1[k] - s0;
+            const uint4 *run = bins + ((long)k * BA + s0) * RW;
+            for (int j = lane; j < c; j += 64)
+                atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
+        }
     }
     __syncthreads();
     // exclusive scan of cur[0..K1): two consecutive columns per thread
@@ -311,19 +352,18 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         const int y0 = 2 * (int)threadIdx.x;
         const int v0 = y0 < K1 ? cur[y0] : 0, v1 = y0 + 1 < K1 ? cur[y0 + 1] : 0;
         const int sm = v0 + v1;
-        int incl = sm;
+        int ci = sm;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
+            const int t = __shfl_up(ci, o);
+            if (lane >= o) ci += t;
         }
-        __syncthreads();  // every thread has read its pair of counts and wtot
-        if (lane == 63) wtot[wv] = incl;
+        if (lane == 63) wcnt[wv] = ci;
         __syncthreads();
-        int woff = 0;
+        int coff = 0;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) woff += k < wv ? wtot[k] : 0;
-        const int e0 = woff + incl - sm;  // start of column y0 within the plane
+        for (int k = 0; k < NW; ++k) coff += k < wv ? wcnt[k] : 0;
+        const int e0 = coff + ci - sm;  // start of column y0 within the plane
         if (y0 < K1) {
             cur[y0] = e0;
             F.colstart[cx * K1 + y0] = pre + e0;
@@ -335,36 +375,33 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + nb;
     }
     __syncthreads();
-    // pass 2: every record to its place, through LDS when the plane fits
-    typedef __attribute__((address_space(1))) uint4 GQ;
-    typedef __attribute__((address_space(3))) uint4 LQ;
-    GQ *dst = (GQ *)F.recs + (long)pre * RW;
-    LQ *sbuf = (LQ *)smem_raw;
-    // (separate loops for the two destinations: with one pointer of either
-    // kind the compiler kept the loaded record in scratch memory)
-    auto place_all = [&](auto *where) {
+    uint4 *dst = reinterpret_cast<uint4 *>(F.recs) + (long)pre * RW;
+    if (staged) {
+        uint4 *sbuf = reinterpret_cast<uint4 *>(smem_raw);
+#pragma unroll
+        for (int q = 0; q < kSortRegs; ++q)
+            if (q * NT + (int)threadIdx.x < nb) {
+                const int slot = atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
+#pragma unroll
+                for (int w = 0; w < RW; ++w) sbuf[slot * RW + w] = r[q][w];
+            }
+        __syncthreads();  // contiguous write-out, 16 bytes per lane
+        for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
+    } else {
         for (int k = wv; k < nbin; k += NW) {
-            const int s0 = bs[(long)k * S + cx], c = bs[(long)k * S + cx + 1] - s0;
-            const GQ *run = (const GQ *)bins + ((long)k * BA + s0) * RW;
+            const int s0 = row0[k], c = row1[k] - s0;
+            const uint4 *run = bins + ((long)k * BA + s0) * RW;
             for (int j = lane; j < c; j += 64) {
                 const uint4 v0 = run[(long)j * RW], v1 = run[(long)j * RW + 1];
                 const uint4 v2 = RW > 2 ? run[(long)j * RW + 2] : v1;
                 // the cell by a load of its own: taken from the loaded words,
                 // the compiler moved the rest of them through scratch memory
-                const int cell = rec_cell<T>((const uint4 *)run, j);
-                const long slot = atomicAdd(&cur[(cell >> 10) & 1023], 1);
-                where[slot * RW] = v0;
-                where[slot * RW + 1] = v1;
-                if (RW > 2) where[slot * RW + 2] = v2;
+                const long slot = atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
+                dst[slot * RW] = v0;
+                dst[slot * RW + 1] = v1;
+                if (RW > 2) dst[slot * RW + 2] = v2;
             }
         }
-    };
-    if (nb <= F.scap) {
-        place_all(sbuf);
-        __syncthreads();  // contiguous write-out, 16 bytes per lane
-        for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
-    } else {
-        place_all(dst);
     }
     if (cx == 0 && F.fixed) {  // the fixed-point scale, from the bin partial maxima
         double m = 0.0;
