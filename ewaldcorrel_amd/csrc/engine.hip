@@ -54,9 +54,14 @@ namespace {
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
-std::vector<int> factor_radices(int N) {
+// Default radix order of a length-N transform: radix 16 first (two
+// register-staged stages for 128 instead of three), then 8, 4, 2, 3, 5 and
+// any other prime; with16 = false leaves out 16 (kernels holding more than
+// 16 elements per thread have no radix-16 stage).
+std::vector<int> factor_radices(int N, bool with16 = true) {
     std::vector<int> r;
     int n = N;
+    while (with16 && n % 16 == 0) { r.push_back(16); n /= 16; }
     while (n % 8 == 0) { r.push_back(8); n /= 8; }
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
@@ -76,7 +81,7 @@ FftPlanArgs plan_args(const FftPlan &p) {
     for (int s = 0; s < a.nstages; ++s) {
         a.radix[s] = p.radix[s];
         const int R = p.radix[s];
-        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) a.generic = 1;
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8 && R != 16) a.generic = 1;
         a.dNR[s] = make_fastdiv((unsigned)(p.N / R));
         a.dNs[s] = make_fastdiv((unsigned)Ns);
         a.twstep[s] = p.N / (Ns * R);
@@ -482,6 +487,12 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                   slab_ept() > 0 &&
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
+        // the larger slab variants have no radix-16 stage
+        if (use_slab && slab_ept() > 16)
+            for (int d = 1; d < 3; ++d) {
+                plan_[d].radix = factor_radices(K[d], false);
+                (d == 1 ? D.plan1 : D.plan2) = plan_args(plan_[d]);
+            }
     }
     // spreading tiles: rows of K2 + order - 1 cells (the z stencil unwrapped),
     // the largest of these that fits; z-chunked tiles are left to the autotuner
@@ -1178,7 +1189,7 @@ unsigned long long Engine::variant() const {
 
 // ---- launch shapes for the autotuner -----------------------------------------------------
 namespace {
-bool plain_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8; }
+bool plain_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16; }
 }  // namespace
 
 TuneCfg Engine::tune_cfg() const {
@@ -1235,7 +1246,7 @@ void Engine::apply_tune(const TuneCfg &c) {
             throw std::invalid_argument("radix stage count");
         long long prod = 1;
         for (int s = 0; s < c.nrad[d]; ++s) {
-            if (!plain_radix(c.radix[d][s])) throw std::invalid_argument("radix not in 2,3,4,5,8");
+            if (!plain_radix(c.radix[d][s])) throw std::invalid_argument("radix not in 2,3,4,5,8,16");
             prod *= c.radix[d][s];
         }
         if (prod != K[d]) throw std::invalid_argument("radices do not multiply to K");
@@ -1250,6 +1261,17 @@ void Engine::apply_tune(const TuneCfg &c) {
         if (!gather_tile_fits(c.gtx, c.gty, c.gtz, K, order, lmax, fp32 ? 4 : 8))
             throw std::invalid_argument("gather tile does not fit");
     if (c.slab && !slab_ok) throw std::invalid_argument("slab transforms do not apply");
+    // radix-16 stages run in kernels with at most 16 elements per thread:
+    // the line kernels always, the y-z slab kernels in their small variant
+    const bool slab_after = c.slab != 0;
+    if (slab_after && slab_ept() > 16)
+        for (int d = 1; d < 3; ++d) {
+            const std::vector<int> &rd = c.nrad[d] ? std::vector<int>(c.radix[d], c.radix[d] + c.nrad[d])
+                                                   : plan_[d].radix;
+            for (int r : rd)
+                if (r == 16)
+                    throw std::invalid_argument("radix 16 does not apply to these slab transforms");
+        }
     FrameDesc &D = desc_;
     for (int d = 0; d < 3; ++d) {
         if (c.nrad[d] == 0) continue;
@@ -1278,7 +1300,7 @@ void Engine::apply_tune(const TuneCfg &c) {
     drop_graph_();
 }
 
-// Every ordered factorisation of K_d into radices 2, 3, 4, 5, 8 with at most
+// Every ordered factorisation of K_d into radices 2, 3, 4, 5, 8, 16 with at most
 // one stage more than the default plan (the default first); at most 32.
 std::vector<std::vector<int>> Engine::radix_options(int d) const {
     const std::vector<int> &def = plan_[d].radix;
@@ -1288,7 +1310,7 @@ std::vector<std::vector<int>> Engine::radix_options(int d) const {
     const int maxlen = std::min<int>((int)def.size() + 1, kMaxRadixStages);
     std::vector<std::vector<int>> all;
     std::vector<int> cur;
-    const int rs[5] = {8, 4, 2, 3, 5};
+    const int rs[6] = {16, 8, 4, 2, 3, 5};
     std::function<void(int)> rec = [&](int rem) {
         if (rem == 1) {
             if (cur != def) all.push_back(cur);

@@ -9,7 +9,7 @@
 // loads use clamped addresses with no branches, so every load of a thread
 // is in flight before the first one is waited for.
 //
-// Radix 2, 3, 4, 5 and 8 have dedicated butterflies; any other prime uses a
+// Radix 2, 3, 4, 5, 8 and 16 have dedicated butterflies; any other prime uses a
 // direct DFT (two LDS buffers for that stage), so every N >= 1 works (the
 // reference accepts any K >= 1; its own test uses K = 13).
 //
@@ -149,6 +149,47 @@ __device__ __forceinline__ void bfly8(cplx<T> *v, int sign) {
     v[7] = csub<T>(e[3], o3);
 }
 
+// radix 16 as 4 x 4: 4-point DFTs over n1 of x[4 n1 + n2], twiddles
+// W16^(n2 k1), 4-point DFTs over n2; outputs in natural order k1 + 4 k2
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul_c(cplx<T> a, T c, T s) {
+    cplx<T> r;
+    r.x = a.x * c - a.y * s;
+    r.y = a.x * s + a.y * c;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ void bfly16(cplx<T> *v, int sign) {
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
+    const T h = T(0.70710678118654752440);
+    const T sg = sign < 0 ? T(-1) : T(1);
+    cplx<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cplx<T> a[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        bfly4<T>(a, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+    }
+    // W16^m = (cos(2 pi m/16), sign sin(2 pi m/16)) for m = n2 k1
+    y[1][1] = cmul_c<T>(y[1][1], c1, sg * s1);   // m = 1
+    y[1][2] = cmul_c<T>(y[1][2], h, sg * h);     // m = 2
+    y[1][3] = cmul_c<T>(y[1][3], s1, sg * c1);   // m = 3
+    y[2][1] = cmul_c<T>(y[2][1], h, sg * h);     // m = 2
+    y[2][2] = mul_si<T>(y[2][2], sign);          // m = 4
+    y[2][3] = cmul_c<T>(y[2][3], -h, sg * h);    // m = 6
+    y[3][1] = cmul_c<T>(y[3][1], s1, sg * c1);   // m = 3
+    y[3][2] = cmul_c<T>(y[3][2], -h, sg * h);    // m = 6
+    y[3][3] = cmul_c<T>(y[3][3], -c1, -sg * s1); // m = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cplx<T> b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        bfly4<T>(b, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+}
+
 // In-place stage of fixed radix R over C lines (line c at buf + c*LS):
 // every thread loads the inputs of up to MAXQ butterflies into registers,
 // transforms them, and after a barrier stores the outputs.  Index arithmetic
@@ -188,6 +229,7 @@ __device__ __forceinline__ void stage_ip(cplx<T> *buf, int C, int LS, int N, int
             else if (R == 4) bfly4<T>(v[q], sign);
             else if (R == 5) bfly5<T>(v[q], sign);
             else if (R == 8) bfly8<T>(v[q], sign);
+            else if (R == 16) bfly16<T>(v[q], sign);
             dst[q] = mul24(c, LS) + mul24(jq, Ns * (R - 1)) + j;
         }
     }
@@ -247,6 +289,10 @@ __device__ __forceinline__ cplx<T> *fft_lines(cplx<T> *b0, cplx<T> *b1, int C, i
             case 4: stage_ip<T, 4, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
             case 5: stage_ip<T, 5, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
             case 8: stage_ip<T, 8, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign); break;
+            case 16:  // one radix-16 butterfly per thread at most (16 registers of values)
+                if constexpr (EPT <= 16)
+                    stage_ip<T, 16, EPT>(cur, C, LS, p.N, Ns, p.dNR[s], p.dNs[s], p.twstep[s], tw, sign);
+                break;
             default: {
                 stage_generic<T>(cur, oth, C, LS, p.N, Ns, R, tw, sign);
                 cplx<T> *t = cur;
@@ -305,6 +351,7 @@ __device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, i
             else if (R == 4) bfly4<T>(v[q], sign);
             else if (R == 5) bfly5<T>(v[q], sign);
             else if (R == 8) bfly8<T>(v[q], sign);
+            else if (R == 16) bfly16<T>(v[q], sign);
             dst[q] = c * LST + (j + jq * Ns * (R - 1)) * EST;
         }
     }
@@ -319,7 +366,7 @@ __device__ __forceinline__ void lstage(cplx<T> *buf, int C, const FastDiv &dC, i
     __syncthreads();
 }
 
-// All stages of a plan whose radices are 2, 3, 4, 5 and 8 (plan.generic == 0).
+// All stages of a plan whose radices are 2, 3, 4, 5, 8 and 16 (plan.generic == 0).
 template <typename T, int EPT, int NT, bool OPQ = false>
 __device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC, int LST,
                                           int EST, const FftPlanArgs &p, const cplx<T> *tw,
@@ -334,7 +381,12 @@ __device__ __forceinline__ void lines_fft(cplx<T> *buf, int C, const FastDiv &dC
             case 3: lstage<T, 3, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
             case 4: lstage<T, 4, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
             case 5: lstage<T, 5, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
-            default: lstage<T, 8, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            case 8: lstage<T, 8, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign); break;
+            default:  // 16: variants with at most 16 elements per thread only (the
+                      // plan is checked on the host)
+                if constexpr (EPT <= 16)
+                    lstage<T, 16, EPT, NT, OPQ>(buf, C, dC, LST, EST, p.N, Ns, dNs, ts, tw, sign);
+                break;
         }
         Ns *= R;
     }

@@ -248,9 +248,9 @@ __global__ void __launch_bounds__((slab_block<T, EPT>()))
         }
     }
     __syncthreads();
-    // per-stage butterfly indices in the larger variants (kept across stages
-    // they are spilled); the small ones run faster keeping them
-    constexpr bool lean = EPT != SlabCfg<T>::small;
+    // per-stage butterfly indices where registers bound the kernel (kept
+    // across stages they are spilled), as in the forward kernel
+    constexpr bool lean = std::is_same<T, float>::value || EPT != SlabCfg<T>::small;
     lines_fft<T, EPT, kSlabBlock, lean>(slab, RS, F.dldim, 1, RS, F.plan1, twy, +1);
     // Z_k = A_k + i B_k and the Hermitian completion (all reads before the barrier)
     {

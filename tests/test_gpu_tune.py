@@ -127,3 +127,35 @@ def test_autotune_on_caller_device_atoms_larger_than_the_initial_buffers():
     assert abs(a[0] - b[0]) <= 1e-6 * abs(a[0])
     assert rel_rms(b[2], a[2]) <= 1e-5 and rel_rms(b[3], a[3]) <= 1e-5
     ew().tune_clear()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("slab", [1, 0])
+def test_radix16_plans_against_the_oracle(precision, slab):
+    """Radix-16 stages (with 2, 3 and 4 after them) on every axis, through
+    the y-z slab or the line passes, imported as remembered shapes."""
+    ew().tune_clear()
+    L6, K, order, beta, w, x = system([31., 29., 33., 1.5, -2., 1.], (32, 48, 64), 4, 2000, 11)
+    s = ew().Sfac(L6, np.array(K), order, precision=precision)
+    s.set_A(ew().ewald_kernel(beta), 1.0)
+    s.autotune(n=2000, reps=1)
+    line = [ln for ln in ew().tune_export().splitlines() if ln.startswith("32x48x64/4/")][0]
+    f = line.split()
+    f[1], f[2], f[3] = "16,2", "16,3", "4,16"
+    f[-1] = str(slab)
+    ew().tune_clear()
+    assert ew().tune_import(" ".join(f)) == 1
+    s2 = ew().Sfac(L6, np.array(K), order, precision=precision)
+    s2.set_A(ew().ewald_kernel(beta), 1.0)
+    E, vir, dx, phi = s2.evaluate(w, x)
+    say("radix-16 shapes", " ".join(f), "E", E)
+    o = OracleSfac(L6, K, order)
+    o.set_A(ewald_rad(beta), 1.0)
+    E0, v0, dx0, phi0 = o.evaluate(w, x)
+    if precision == "fp64":
+        assert abs(E - E0) <= 1e-10 * abs(E0) and rel(vir, v0) <= 1e-10
+        assert rel_rms(dx, dx0) <= 1e-9 and rel_rms(phi, phi0) <= 1e-9
+    else:
+        assert abs(E - E0) <= 1e-6 * abs(E0) and rel(vir, v0) <= 1e-5
+        assert rel_rms(dx, dx0) <= 1e-5 and rel_rms(phi, phi0) <= 1e-5
+    ew().tune_clear()
