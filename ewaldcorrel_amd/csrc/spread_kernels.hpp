@@ -227,13 +227,14 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
 
 // ---- sort: one workgroup per (frame, x plane) ------------------------------------
 // Counting sort of the plane's records by column.  The plane's records are
-// its runs in the bin chunks.  Staged (the plane fits F.scap records, at most
-// kSortRegs per thread, at most kSortThreads chunks): a table of the runs in
-// LDS maps record i of the plane to its chunk, each thread loads its records
-// into registers once, counts their columns, and places them in column order
-// in LDS, written out contiguously.  Otherwise each run is read by one
-// wavefront, once to count and once to place the records straight into
-// their place.  Column starts (offset by the atoms of all earlier planes)
+// its runs in the bin chunks; with at most 2 kSortThreads chunks a map in LDS
+// gives the chunk index of each record of a window of the plane (the threads
+// load consecutive records, all lanes busy).  Staged (the plane fits F.scap
+// records and one window): each thread loads its records into registers
+// once, counts their columns, and places them in column order in LDS,
+// written out contiguously.  Larger planes: window by window, once to count
+// and once to place each record straight into its place.  More chunks: each
+// run is read by one wavefront, the same two passes.  Column starts (offset by the atoms of all earlier planes)
 // are written either way.  The order within a column is the order of the
 // cursor atomics; no result depends on it (fixed-point sums are exact
 // integers and every atom's gather is its own).
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     constexpr int NT = kSortThreads, NW = kSortThreads / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int cur[kMaxK];  // column counts, then their starts, then cursors
-    __shared__ int rmap[NT * kSortRegs];  // staged: bins index of the plane's record i
+    __shared__ int rmap[NT * kSortRegs];  // bins index of the plane's record i (a window)
     __shared__ int wtot[NW];
     __shared__ int wcnt[NW];
     __shared__ double wred[NW];
@@ -284,20 +285,31 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     const uint4 *bins = reinterpret_cast<const uint4 *>(F.bins);
     // atoms of the planes before this one (part), this plane's run in chunk
     // k = thread (c0, when there are at most NT chunks) and the runs' wave sums
-    int part = 0, c0 = 0, s00 = 0;
-    if (nbin <= NT) {
-        if ((int)threadIdx.x < nbin) {
-            s00 = row0[threadIdx.x];
-            c0 = row1[threadIdx.x] - s00;
+    // table mode (at most 2 NT chunks): thread t owns the plane's runs in
+    // chunks t and t + NT, placed one after the other at ex in the plane's
+    // record order (any order of the runs will do: the records are sorted by
+    // column afterwards); otherwise a wavefront per run
+    const bool table = nbin <= 2 * NT;
+    const int ka = threadIdx.x, kb = threadIdx.x + NT;
+    int part = 0, ca = 0, cb = 0, sa = 0, sb = 0;
+    if (table) {
+        if (ka < nbin) {
+            sa = row0[ka];
+            ca = row1[ka] - sa;
         }
-        part = s00;
+        if (kb < nbin) {
+            sb = row0[kb];
+            cb = row1[kb] - sb;
+        }
+        part = sa + sb;
     } else {
         for (int k = threadIdx.x; k < nbin; k += blockDim.x) {
             const int s0 = row0[k];
             part += s0;
-            c0 += row1[k] - s0;
+            ca += row1[k] - s0;
         }
     }
+    const int c0 = ca + cb;
     int incl = c0;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -318,13 +330,20 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         nb += wcnt[k];
         woff += k < wv ? wcnt[k] : 0;
     }
-    const bool staged = nb <= F.scap && nb <= NT * kSortRegs && nbin <= NT;
+    const int ex = woff + incl - c0;
+    const int bega = ka * BA + sa, begb = kb * BA + sb;
+    constexpr int WIN = NT * kSortRegs;  // records of one window of the map
+    // the map of the plane's records [i0, i0 + WIN): the bins index of each
+    // (every thread writes the parts of its runs inside the window)
+    auto fill_map = [&](int i0) {
+        for (int j = max(ex, i0); j < min(ex + ca, i0 + WIN); ++j) rmap[j - i0] = bega + (j - ex);
+        for (int j = max(ex + ca, i0); j < min(ex + c0, i0 + WIN); ++j)
+            rmap[j - i0] = begb + (j - ex - ca);
+    };
+    const bool staged = table && nb <= F.scap && nb <= WIN;
     uint4 r[kSortRegs][RW];
     if (staged) {
-        // the map of the plane's records: thread k writes the bins indices of
-        // chunk k's run at the run's place in the plane
-        const int ex = woff + incl - c0, beg = (int)threadIdx.x * BA + s00;
-        for (int j = 0; j < c0; ++j) rmap[ex + j] = beg + j;
+        fill_map(0);
         __syncthreads();
         // this thread's records into registers (all loads in flight together)
 #pragma unroll
@@ -337,6 +356,18 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
             if (q * NT + (int)threadIdx.x < nb) atomicAdd(&cur[(raw_cell<T>(r[q]) >> 10) & 1023], 1);
+    } else if (table) {
+        // column counts, window by window
+        for (int i0 = 0; i0 < nb; i0 += WIN) {
+            fill_map(i0);
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kSortRegs; ++q) {
+                const int i = i0 + q * NT + (int)threadIdx.x;
+                if (i < nb) atomicAdd(&cur[(rec_cell<T>(bins, rmap[i - i0]) >> 10) & 1023], 1);
+            }
+            __syncthreads();  // the map is rewritten for the next window
+        }
     } else {
         // column counts, a wavefront per run (chunks wv, wv + NW, ...)
         for (int k = wv; k < nbin; k += NW) {
@@ -387,6 +418,27 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             }
         __syncthreads();  // contiguous write-out, 16 bytes per lane
         for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
+    } else if (table) {
+        // every record straight to its place, window by window
+        for (int i0 = 0; i0 < nb; i0 += WIN) {
+            fill_map(i0);
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kSortRegs; ++q) {
+                const int i = i0 + q * NT + (int)threadIdx.x;
+                if (i < nb) {
+                    const long g = rmap[i - i0];
+                    const uint4 v0 = bins[g * RW], v1 = bins[g * RW + 1];
+                    const uint4 v2 = RW > 2 ? bins[g * RW + 2] : v1;
+                    // the cell by a load of its own (as below)
+                    const long slot = atomicAdd(&cur[(rec_cell<T>(bins, (int)g) >> 10) & 1023], 1);
+                    dst[slot * RW] = v0;
+                    dst[slot * RW + 1] = v1;
+                    if (RW > 2) dst[slot * RW + 2] = v2;
+                }
+            }
+            __syncthreads();  // the map is rewritten for the next window
+        }
     } else {
         for (int k = wv; k < nbin; k += NW) {
             const int s0 = row0[k], c = row1[k] - s0;
