@@ -525,6 +525,7 @@ STAGE_OF_KERNEL = {
     "spme::fft_z_r2c_kernel": "fft_z_r2c",
     "spme::fft_y_kernel": "fft_y",
     "spme::fft_x_kernel<float, 2": "fft_x_recip",
+    "spme::fft_x_reg_kernel<float": "fft_x_recip",
     "spme::fft_yz_inv_kernel": "fft_yz_inv",
     "spme::fft_z_c2r_kernel": "fft_z_c2r",
     "spme::gather_kernel": "gather_sorted",

@@ -46,7 +46,7 @@ const char *stage_name(Stage s) {
         "bin",       "sort",      "spread",    "fft_z_r2c",
         "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
         "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
-        "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile"};
+        "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile", "fft_x_recip"};
     return names[s];
 }
 
@@ -185,6 +185,10 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
         case ST_XFUSED:
             launch_x<T, 2>(L, g, bl, lds, st, -1, 0);
             break;
+        case ST_XREG:
+            if constexpr (std::is_same<T, float>::value)
+                fft_x_reg_kernel<T><<<g, kXRegThreads, lds, st>>>(L.frames, L.offs, L.nf);
+            break;
         case ST_YZ_FWD:
         case ST_YZ_INV: {
             auto go = [&](auto ept_c) {
@@ -243,7 +247,7 @@ void launch_order(int order, Stage s, const Launch &L, int blocks,
 // the reduction of a stage is finalized by the next launch of the sequence:
 // 1 after the fused x pass, 2 after the standalone influence launch
 int finishes(Stage prev, Stage s) {
-    if (prev == ST_XFUSED && (s == ST_YI || s == ST_YZ_INV)) return 1;
+    if ((prev == ST_XFUSED || prev == ST_XREG) && (s == ST_YI || s == ST_YZ_INV)) return 1;
     if (prev == ST_INFL && s == ST_XI) return 2;
     return 0;
 }
@@ -254,13 +258,13 @@ const Stage kDe1Lines[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
 const Stage kDe1Slab[] = {ST_INFL, ST_XI, ST_YZ_INV};
 const Stage kEnStages[] = {ST_INFL_E, ST_FINISH};
 const Stage kEvalLines[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
-                            ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_SORTED};
+                            ST_YF,   ST_XFUSED, ST_XREG, ST_YI, ST_ZC2R, ST_GATHER_SORTED};
 const Stage kEvalSlab[] = {ST_BIN, ST_SORT, ST_SPREAD,
-                           ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_SORTED};
+                           ST_YZ_FWD, ST_XFUSED, ST_XREG, ST_YZ_INV, ST_GATHER_SORTED};
 const Stage kEvalLinesTile[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
-                                ST_YF,   ST_XFUSED, ST_YI,      ST_ZC2R,   ST_GATHER_TILE};
+                                ST_YF,   ST_XFUSED, ST_XREG, ST_YI, ST_ZC2R, ST_GATHER_TILE};
 const Stage kEvalSlabTile[] = {ST_BIN, ST_SORT, ST_SPREAD,
-                               ST_YZ_FWD, ST_XFUSED, ST_YZ_INV,  ST_GATHER_TILE};
+                               ST_YZ_FWD, ST_XFUSED, ST_XREG, ST_YZ_INV, ST_GATHER_TILE};
 
 template <size_t NS> StageSeq seq_of(const Stage (&a)[NS]) { return StageSeq{a, (int)NS}; }
 
@@ -812,6 +816,8 @@ bool Engine::uses_stage(Stage s) const {
         case ST_ZC2R: return !slab();
         case ST_GATHER_TILE: return gtile();
         case ST_GATHER_SORTED: return !gtile();
+        case ST_XFUSED: return !xreg();
+        case ST_XREG: return xreg();
         default: return true;
     }
 }
@@ -827,8 +833,9 @@ int Engine::stage_blocks_n(Stage s, int natoms) const {
         case ST_ZC2R: return (nrows + 2 * D.zC - 1) / (2 * D.zC);
         case ST_YF:
         case ST_YI: return K[0] * D.yT;
+        case ST_XFUSED: return xreg() ? 0 : D.xB;
+        case ST_XREG: return xreg() ? D.xB : 0;
         case ST_XF_SCALE:
-        case ST_XFUSED:
         case ST_XI: return D.xB;
         case ST_YZ_FWD:
         case ST_YZ_INV: return K[0];
@@ -869,6 +876,7 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
         case ST_XI:
             return fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
                         : xlines_lds_bytes<double>(K[0], D.xC, xgen);
+        case ST_XREG: return xreg_lds_bytes<float>();
         case ST_XFUSED:  // the lines, the wavefront sums, the closed form's x factors
             return (fp32 ? xlines_lds_bytes<float>(K[0], D.xC, xgen)
                          : xlines_lds_bytes<double>(K[0], D.xC, xgen)) +
@@ -1171,6 +1179,12 @@ void Engine::download_results(int n, double *out, double *dx, double *phi, hipSt
                             st));
 }
 
+bool Engine::xreg() const {
+    const std::vector<int> &r = plan_[0].radix;
+    return fp32 && K[0] == kXRegN && r.size() == 2 && r[0] == 16 && r[1] == 8 &&
+           desc_.xC == kXRegC && x_influence_form(desc_) == kXInfSeparable;
+}
+
 unsigned long long Engine::variant() const {
     unsigned long long v = 0;
     v |= (unsigned long long)(fp32 ? 1 : 0);
@@ -1184,6 +1198,7 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(desc_.ntz > 1 ? 1 : 0) << 48;
     v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
     v |= (unsigned long long)(x_influence_form(desc_) & 3) << 50;
+    v |= (unsigned long long)(xreg() ? 1 : 0) << 52;
     return v;
 }
 
@@ -1836,7 +1851,13 @@ void Batch::refresh_variant_() {
     seq_.assign({ST_BIN, ST_SORT, ST_SPREAD});
     if (any_slab) seq_.push_back(ST_YZ_FWD);
     if (any_lines) seq_.insert(seq_.end(), {ST_ZR2C, ST_YF});
-    seq_.push_back(ST_XFUSED);
+    bool any_xreg = false, any_xfused = false;
+    for (const Engine *e : frames_) {
+        any_xreg = any_xreg || e->xreg();
+        any_xfused = any_xfused || !e->xreg();
+    }
+    if (any_xfused) seq_.push_back(ST_XFUSED);
+    if (any_xreg) seq_.push_back(ST_XREG);
     if (any_slab) seq_.push_back(ST_YZ_INV);
     if (any_lines) seq_.insert(seq_.end(), {ST_YI, ST_ZC2R});
     if (any_tile) seq_.push_back(ST_GATHER_TILE);

@@ -32,7 +32,7 @@ enum EngineFlags {
 enum Stage {
     ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
     ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
-    ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_COUNT
+    ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_XREG, ST_COUNT
 };
 
 // stage sequences of the entry points for the slab / line transforms
@@ -156,6 +156,9 @@ public:
     // sizes); a batch of engines with one signature runs each frame through
     // exactly the kernels it runs alone
     unsigned long long variant() const;
+    // the fused x pass runs the register kernel (FP32, K0 = 128 with the x
+    // plan 16, 8, the FP64 influence of an x-separable box)
+    bool xreg() const;
 
     // autotuning: the current launch shapes, applying others (validated
     // against this grid; std::invalid_argument if one does not apply), and

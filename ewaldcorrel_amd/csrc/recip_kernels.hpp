@@ -694,6 +694,52 @@ __device__ __forceinline__ void xsep_column(const FrameDesc &F, int ky, int kz, 
     k[5] = L.mult;
 }
 
+// One point kx = e of an x line of the x-separable form: S-hat = v iB (in
+// FP64), the images of each group with their multiplicities and sign sums
+// added to L, and v A iB written back (A holds the line's mult / 2).
+template <typename T>
+__device__ __forceinline__ void xsep_point(XLine64 &L, const FrameDesc &F, const double *gx,
+                                           const double *k, double q00, bool both, int ng,
+                                           int kz, int e, double ib0e, cplx<T> &v) {
+    const int K0 = F.K0;
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac, max_m2 = F.max_m2;
+    const double ib = ib0e * k[4];
+    const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
+    const double s = re * re + im * im;
+    const int nx = e <= K0 / 2 ? e : e - K0;
+    const bool xny = (K0 % 2) == 0 && e == K0 / 2;
+    double a = 0.0;
+#pragma unroll 1
+    for (int gi = 0; gi < ng; ++gi) {
+        // group g: the image checked is (nx, sy, sz) = (nx, 1, 1), or for
+        // g = -1 (nx, -1, 1) on a y Nyquist line, else (nx, 1, -1)
+        const int g = gi ? -1 : 1;
+        const int sy = L.yny ? g : 1, sz = L.yny ? 1 : g;
+        double m2;
+        if (!L.in_ball(cl, q00, max_m2, nx, sy, sz, kz, k[gi], 0.0, m2)) continue;
+        const double r = recip_f64(m2);
+        const double f = gx[nx < 0 ? -nx : nx] * k[2 + gi] * r;
+        const double w = k[5] * s * ((fac - r) * f);
+        const double nxd = nx;
+        // multiplicity xf np of the group's images, and w times the sums
+        // of nxs sy and nxs sz over them
+        const int mul = (xny ? 2 : 1) * (both ? 2 : 1);
+        const int ssy = (xny || both) ? 0 : nx * sy, ssz = (xny || both) ? 0 : nx * sz;
+        const double mw = (double)mul * w;
+        L.W0 += mw;
+        L.W2 = fma(mw * nxd, nxd, L.W2);
+        L.Wy = fma(w, (double)ssy, L.Wy);
+        L.Wz = fma(w, (double)ssz, L.Wz);
+        L.Wyz = gi ? L.Wyz - mw : L.Wyz + mw;
+        a = fma((double)mul, f, a);
+    }
+    L.aE = fma(a, s, L.aE);
+    const double tt = a * ib;
+    v.x = T(re * tt);
+    v.y = T(im * tt);
+}
+
 template <typename T>
 __device__ __forceinline__ void x_influence64_xsep(const FrameDesc &F, cplx<T> *res,
                                                    const double *ib0, const double *gx,
@@ -715,41 +761,8 @@ __device__ __forceinline__ void x_influence64_xsep(const FrameDesc &F, cplx<T> *
     for (int q = 0; q < EPT; ++q) {
         const int e = e0 + q * es;
         if (e >= N || !valid) continue;
-        const double ib = ib0[e] * k[4];
         cplx<T> v = res[c * LS + e];
-        const double re = (double)v.x * ib, im = (double)v.y * ib;  // S-hat
-        const double s = re * re + im * im;
-        const int nx = e <= K0 / 2 ? e : e - K0;
-        const bool xny = (K0 % 2) == 0 && e == K0 / 2;
-        double a = 0.0;
-#pragma unroll 1
-        for (int gi = 0; gi < ng; ++gi) {
-            // group g: the image checked is (nx, sy, sz) = (nx, 1, 1), or for
-            // g = -1 (nx, -1, 1) on a y Nyquist line, else (nx, 1, -1)
-            const int g = gi ? -1 : 1;
-            const int sy = L.yny ? g : 1, sz = L.yny ? 1 : g;
-            double m2;
-            if (!L.in_ball(cl, q00, max_m2, nx, sy, sz, kz, k[gi], 0.0, m2)) continue;
-            const double r = recip_f64(m2);
-            const double f = gx[nx < 0 ? -nx : nx] * k[2 + gi] * r;
-            const double w = k[5] * s * ((fac - r) * f);
-            const double nxd = nx;
-            // multiplicity xf np of the group's images, and w times the sums
-            // of nxs sy and nxs sz over them
-            const int mul = (xny ? 2 : 1) * (both ? 2 : 1);
-            const int ssy = (xny || both) ? 0 : nx * sy, ssz = (xny || both) ? 0 : nx * sz;
-            const double mw = (double)mul * w;
-            L.W0 += mw;
-            L.W2 = fma(mw * nxd, nxd, L.W2);
-            L.Wy = fma(w, (double)ssy, L.Wy);
-            L.Wz = fma(w, (double)ssz, L.Wz);
-            L.Wyz = gi ? L.Wyz - mw : L.Wyz + mw;
-            a = fma((double)mul, f, a);
-        }
-        L.aE = fma(a, s, L.aE);
-        const double tt = a * ib;
-        v.x = T(re * tt);
-        v.y = T(im * tt);
+        xsep_point<T>(L, F, gx, k, q00, both, ng, kz, e, ib0[e], v);
         res[c * LS + e] = v;
     }
     L.finish(cl, kz, wsums);
@@ -975,6 +988,143 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((x_
             }
             col[(long)e * P] = v;
         }
+    }
+}
+
+// ---- register x pass: 128-point lines as 16 x 8 --------------------------------------------
+// The fused x pass (forward, /B, FP64 influence of x-separable boxes, inverse)
+// of FP32 frames whose x plan is radix 16 then 8 (N = 128), with the 16- and
+// 8-point transforms of each line in registers and two exchanges through LDS
+// in place of the LDS-staged stages (four LDS accesses per element instead of
+// twelve, two workgroup barriers instead of ten).  Thread (c, j) = (t mod 16,
+// t div 16) of 128 owns column c of the workgroup's 16 consecutive (ky, kz)
+// columns (each x element a 128-B run in HBM):
+//   forward: x[j + 8 r] (r < 16) loaded from HBM; 16-point DFT over r;
+//            twiddle W_128^(j k1); exchange; the thread takes k1 = 2j, 2j+1:
+//            8-point DFT over j -> X[k1 + 16 k2] (k2 < 8)
+//   influence on its 16 points, per point as the LDS-staged pass
+//   inverse: 8-point inverse DFT over k2 (output index j'), twiddle
+//            W_128^-(j' k1); exchange; 16-point inverse DFT over k1 ->
+//            x[j + 8 r], stored.
+// The forward transform performs the operations of the LDS-staged plan
+// (16, 8); the inverse is decomposed the other way (k = k1 + 16 k2), so its
+// rounding differs from the LDS-staged inverse's.  One partial of E and the
+// virial per workgroup, as the LDS-staged pass.
+constexpr int kXRegThreads = 128;
+constexpr int kXRegN = 128;
+constexpr int kXRegC = 16;    // columns per workgroup
+constexpr int kXRegLS = 129;  // column stride of the exchange buffer (complex)
+template <typename T> __host__ __device__ inline size_t xreg_lds_bytes() {
+    return (size_t)kXRegN * sizeof(cplx<T>) + (size_t)kXRegN * sizeof(double) +
+           (size_t)kXRegC * kXRegLS * sizeof(cplx<T>) +
+           (size_t)(kXRegThreads / 64) * 8 * sizeof(double) +
+           (xfused_gx_doubles(kXRegN) + (size_t)kXColConsts * kXRegC) * sizeof(double);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kXRegThreads) fft_x_reg_kernel(SPME_FRAME_ARGS) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int N = kXRegN, C = kXRegC, LS = kXRegLS;
+    int b;
+    const FrameDesc &F = frames[find_frame(offs, nf, b)];
+    const int ldim = F.ldim, P = F.K1 * ldim;
+    cplx<T> *twl = reinterpret_cast<cplx<T> *>(smem_raw);
+    double *ib0 = reinterpret_cast<double *>(twl + N);
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(ib0 + N);
+    double *wsums = reinterpret_cast<double *>(buf + C * LS);
+    double *gx = wsums + (kXRegThreads / 64) * 8;
+    double *colk = gx + xfused_gx_doubles(N);
+    const int t = threadIdx.x, c = t & (C - 1), j = t >> 4;
+    const int f = b * C + c;
+    const bool valid = f < P;
+    const int fc = valid ? f : P - 1;  // clamped for loads
+    cplx<T> *col = (cplx<T> *)F.grid + fc;
+    cplx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = col[(long)(j + 8 * r) * P];
+    {
+        const cplx<T> *tw = (const cplx<T> *)F.tw0;
+        for (int e = t; e < N; e += kXRegThreads) {
+            twl[e] = tw[e];
+            ib0[e] = F.iB0[e];
+        }
+        const double q00 = F.tcell.iL00 * F.tcell.iL00;
+        for (int e = t; e <= N / 2; e += kXRegThreads) {
+            const double nx = e;
+            gx[e] = exp_nonpos(F.ewald_fac * q00 * nx * nx);
+        }
+        if (t < C) {
+            const int fk = min(b * C + t, P - 1);
+            const int ky = fdiv(fk, F.dldim);
+            xsep_column(F, ky, fk - ky * ldim, colk + kXColConsts * t);
+        }
+    }
+    __syncthreads();
+    // forward: 16-point DFT over r, twiddles, exchange
+    bfly16<T>(v, -1);
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul<T>(v[k1], twiddle<T>(twl, j * k1, -1));
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) buf[c * LS + k1 * 8 + j] = v[k1];
+    __syncthreads();
+    cplx<T> u[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) u[h][jj] = buf[c * LS + (2 * j + h) * 8 + jj];
+    bfly8<T>(u[0], -1);
+    bfly8<T>(u[1], -1);
+    // influence of the thread's 16 points X[k1 + 16 k2]
+    {
+        const int ky = fdiv(fc, F.dldim), kz = fc - ky * ldim;
+        const CellArgs &cl = F.tcell;
+        const double q00 = cl.iL00 * cl.iL00 + cl.iL10 * cl.iL10 + cl.iL20 * cl.iL20;
+        XLine64 L(F, ky, kz);
+        const bool both = L.yny && L.zn2;
+        const int ng = (L.yny || L.zn2) ? 2 : 1;
+        const double *k = colk + kXColConsts * c;
+        if (valid) {
+            // one point per iteration from the head of the 16 values, put back
+            // at their tail (a rotation keeps every index static, so they stay
+            // in registers; after 16 iterations they are in order again)
+#pragma unroll 1
+            for (int i = 0; i < 16; ++i) {
+                const int e = 2 * j + (i >> 3) + 16 * (i & 7);
+                cplx<T> p = u[0][0];
+                xsep_point<T>(L, F, gx, k, q00, both, ng, kz, e, ib0[e], p);
+#pragma unroll
+                for (int q = 0; q < 15; ++q) u[q >> 3][q & 7] = u[(q + 1) >> 3][(q + 1) & 7];
+                u[1][7] = p;
+            }
+        }
+        L.finish(cl, kz, wsums);
+    }
+    // inverse: 8-point inverse DFT over k2, twiddles, exchange (each thread
+    // writes the places it read: no barrier needed before)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k1 = 2 * j + h;
+        bfly8<T>(u[h], +1);
+#pragma unroll
+        for (int jj = 1; jj < 8; ++jj) u[h][jj] = cmul<T>(u[h][jj], twiddle<T>(twl, jj * k1, +1));
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) buf[c * LS + k1 * 8 + jj] = u[h][jj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = buf[c * LS + k1 * 8 + j];
+    bfly16<T>(v, +1);
+    if (valid) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) col[(long)(j + 8 * r) * P] = v[r];
+    }
+    // the workgroup's partial: its wavefront sums in order (made visible by
+    // the barrier above)
+    if (t < 7) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < kXRegThreads / 64; ++q) sum += wsums[q * 8 + t];
+        F.partials[b * 8 + t] = sum;
     }
 }
 
