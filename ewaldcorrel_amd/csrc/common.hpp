@@ -19,7 +19,7 @@ namespace spme {
 constexpr int kMaxOrder = 12;      // largest even B-spline order compiled in
 constexpr int kMaxRadixStages = 16;
 constexpr int kBlock = 256;        // threads per workgroup (spreading uses kSpreadBlock)
-constexpr int kSpreadBlock = 1024;
+constexpr int kSpreadBlock = 512;
 constexpr int kMaxK = 1024;        // largest grid dimension supported
 
 template <typename T> struct cplx_t;
