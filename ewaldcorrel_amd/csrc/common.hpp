@@ -91,17 +91,18 @@ enum TableMode {
     kTableAnalytic = 1,  // closed-form Ewald A and dA from the indices (no table)
 };
 
-// One atom in column order, written once by the scatter and read coalesced
-// by the spreading and the sorted gather: fraction f = u - c of u = s K in
-// the working precision, charge, cell c packed as cx << 20 | cy << 10 | cz
-// (K <= 1024), and the atom's index in the caller's arrays.
+// One atom in column order, written once by the sort and read coalesced by
+// the spreading and the gathers: fraction f = u - c of u = s K in the working
+// precision, cell c packed as cx << 20 | cy << 10 | cz (K <= 1024), charge
+// (FP32 records hold it in FP32, as the FP32 engine spreads and gathers it;
+// FP64 records in FP64), and the atom's index in the caller's arrays.  Copied
+// as 8-byte words (rec_words).
 template <typename T> struct AtomRec;
-template <> struct __align__(32) AtomRec<float> {
+template <> struct __align__(8) AtomRec<float> {
     float f[3];
     int cell;
-    double w;
+    float w;
     int perm;
-    int pad;
 };
 template <> struct __align__(16) AtomRec<double> {
     double f[3];
@@ -110,7 +111,7 @@ template <> struct __align__(16) AtomRec<double> {
     int perm;
     int pad[2];
 };
-static_assert(sizeof(AtomRec<float>) == 32, "AtomRec<float> is 32 bytes");
+static_assert(sizeof(AtomRec<float>) == 24, "AtomRec<float> is 24 bytes");
 static_assert(sizeof(AtomRec<double>) == 48, "AtomRec<double> is 48 bytes");
 
 // a box whose x axis separates from y and z in reciprocal space (iL10 = iL20

@@ -95,26 +95,36 @@ __device__ __forceinline__ double fixed_scale(int n, double wmax, double mmax3, 
 }
 
 // the record of atom a (scale_wrap, cell and fraction as atom_cell) as raw
-// 16-byte words in the layout of AtomRec<T>
+// 8-byte words in the layout of AtomRec<T>
+template <typename T> __host__ __device__ constexpr int rec_words() {
+    return (int)(sizeof(AtomRec<T>) / sizeof(uint2));
+}
+
 template <typename T>
-__device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx, uint4 *v) {
+__device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx, uint2 *v) {
     int c[3];
     double f[3];
     atom_cell(F, a, c, f);
     cx = c[0];
     const unsigned cell = (unsigned)((c[0] << 20) | (c[1] << 10) | c[2]);
-    const unsigned long long wb = (unsigned long long)__double_as_longlong(F.w[a]);
     if constexpr (std::is_same<T, float>::value) {
-        v[0] = make_uint4(__float_as_uint((float)f[0]), __float_as_uint((float)f[1]),
-                          __float_as_uint((float)f[2]), cell);
-        v[1] = make_uint4((unsigned)wb, (unsigned)(wb >> 32), (unsigned)a, 0u);
+        static_assert(offsetof(AtomRec<float>, cell) == 12 && offsetof(AtomRec<float>, w) == 16,
+                      "AtomRec<float> layout");
+        v[0] = make_uint2(__float_as_uint((float)f[0]), __float_as_uint((float)f[1]));
+        v[1] = make_uint2(__float_as_uint((float)f[2]), cell);
+        v[2] = make_uint2(__float_as_uint((float)F.w[a]), (unsigned)a);
     } else {
-        const unsigned long long f0 = (unsigned long long)__double_as_longlong(f[0]);
-        const unsigned long long f1 = (unsigned long long)__double_as_longlong(f[1]);
-        const unsigned long long f2 = (unsigned long long)__double_as_longlong(f[2]);
-        v[0] = make_uint4((unsigned)f0, (unsigned)(f0 >> 32), (unsigned)f1, (unsigned)(f1 >> 32));
-        v[1] = make_uint4((unsigned)f2, (unsigned)(f2 >> 32), (unsigned)wb, (unsigned)(wb >> 32));
-        v[2] = make_uint4(cell, (unsigned)a, 0u, 0u);
+        static_assert(offsetof(AtomRec<double>, w) == 24 && offsetof(AtomRec<double>, cell) == 32,
+                      "AtomRec<double> layout");
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const unsigned long long fb = (unsigned long long)__double_as_longlong(f[d]);
+            v[d] = make_uint2((unsigned)fb, (unsigned)(fb >> 32));
+        }
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(F.w[a]);
+        v[3] = make_uint2((unsigned)wb, (unsigned)(wb >> 32));
+        v[4] = make_uint2(cell, (unsigned)a);
+        v[5] = make_uint2(0u, 0u);
     }
 }
 
@@ -163,7 +173,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *wt, int *total) 
 template <typename T>
 __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
     constexpr int BA = bin_atoms<T>(), A = BA / kBinThreads;
-    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    constexpr int RW = rec_words<T>();
     extern __shared__ __align__(16) unsigned char smem_raw[];  // BA records in plane order
     __shared__ int pl[kMaxK];  // per-plane count, then the plane's start in the chunk
     __shared__ int wt[kBinThreads / 64];
@@ -173,7 +183,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
     const int K0 = F.K0;
     for (int p = threadIdx.x; p < K0; p += blockDim.x) pl[p] = 0;
     __syncthreads();
-    uint4 r[A][RW];
+    uint2 r[A][RW];
     int cx[A], lr[A];
     double wm = 0.0;
 #pragma unroll
@@ -182,7 +192,8 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         cx[q] = -1;
         if (a < F.n) {
             make_rec_raw<T>(F, a, cx[q], r[q]);
-            wm = fmax(wm, fabs(F.w[a]));
+            // the charge as the record holds it (FP32 records: rounded to FP32)
+            wm = fmax(wm, fabs((double)(T)F.w[a]));
             lr[q] = atomicAdd(&pl[cx[q]], 1);
         }
     }
@@ -200,7 +211,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         if (p == 0) F.binstart[(long)K0 * nbin + b] = tot;
     }
     __syncthreads();
-    uint4 *stage = reinterpret_cast<uint4 *>(smem_raw);
+    uint2 *stage = reinterpret_cast<uint2 *>(smem_raw);
 #pragma unroll
     for (int q = 0; q < A; ++q) {
         if (cx[q] < 0) continue;
@@ -210,7 +221,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
     }
     __syncthreads();
     const int nb = min(BA, F.n - b * BA);
-    uint4 *dst = reinterpret_cast<uint4 *>(F.bins) + (long)b * BA * RW;
+    uint2 *dst = reinterpret_cast<uint2 *>(F.bins) + (long)b * BA * RW;
     for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = stage[i];
     if (F.fixed) {  // one partial maximum per workgroup
         for (int off = 32; off > 0; off >>= 1) wm = fmax(wm, __shfl_xor(wm, off));
@@ -241,19 +252,19 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
 constexpr int kSortThreads = 512;
 constexpr int kSortRegs = 4;  // records per thread held in registers (staged sort)
 
-// the packed cell of a record held as raw 16-byte words
-template <typename T> __device__ __forceinline__ int raw_cell(const uint4 *v);
-template <> __device__ __forceinline__ int raw_cell<float>(const uint4 *v) {
-    static_assert(offsetof(AtomRec<float>, cell) == 12, "cell is word 3 of the first quad");
-    return (int)v[0].w;
+// the packed cell of a record held as raw 8-byte words
+template <typename T> __device__ __forceinline__ int raw_cell(const uint2 *v);
+template <> __device__ __forceinline__ int raw_cell<float>(const uint2 *v) {
+    static_assert(offsetof(AtomRec<float>, cell) == 12, "cell is word 1, high half");
+    return (int)v[1].y;
 }
-template <> __device__ __forceinline__ int raw_cell<double>(const uint4 *v) {
-    static_assert(offsetof(AtomRec<double>, cell) == 32, "cell is word 0 of the third quad");
-    return (int)v[2].x;
+template <> __device__ __forceinline__ int raw_cell<double>(const uint2 *v) {
+    static_assert(offsetof(AtomRec<double>, cell) == 32, "cell is word 4, low half");
+    return (int)v[4].x;
 }
 // the packed cell of record i of a chunk, loaded alone
-template <typename T> __device__ __forceinline__ int rec_cell(const uint4 *chunk, int i) {
-    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+template <typename T> __device__ __forceinline__ int rec_cell(const uint2 *chunk, int i) {
+    constexpr int RW = rec_words<T>();
     typedef __attribute__((address_space(1))) const int GI;
     GI *wd = (GI *)(chunk + (long)i * RW);
     return std::is_same<T, float>::value ? wd[3] : wd[8];
@@ -267,7 +278,7 @@ template <typename T> __host__ __device__ inline size_t sort_lds_bytes(int scap)
 template <typename T>
 __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     constexpr int BA = bin_atoms<T>();
-    constexpr int RW = sizeof(AtomRec<T>) / sizeof(uint4);
+    constexpr int RW = rec_words<T>();
     constexpr int NT = kSortThreads, NW = kSortThreads / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int cur[kMaxK];  // column counts, then their starts, then cursors
@@ -282,7 +293,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     const int nbin = (F.n + BA - 1) / BA;
     // the plane's start and length in every chunk: rows cx and cx + 1
     const int *row0 = F.binstart + (long)cx * nbin, *row1 = row0 + nbin;
-    const uint4 *bins = reinterpret_cast<const uint4 *>(F.bins);
+    const uint2 *bins = reinterpret_cast<const uint2 *>(F.bins);
     // atoms of the planes before this one (part), this plane's run in chunk
     // k = thread (c0, when there are at most NT chunks) and the runs' wave sums
     // table mode (at most 2 NT chunks): thread t owns the plane's runs in
@@ -341,7 +352,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             rmap[j - i0] = begb + (j - ex - ca);
     };
     const bool staged = table && nb <= F.scap && nb <= WIN;
-    uint4 r[kSortRegs][RW];
+    uint2 r[kSortRegs][RW];
     if (staged) {
         fill_map(0);
         __syncthreads();
@@ -351,7 +362,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             const int i = q * NT + (int)threadIdx.x;
             const long g = i < nb ? rmap[i] : 0;
 #pragma unroll
-            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bins[g * RW + w] : make_uint4(0, 0, 0, 0);
+            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bins[g * RW + w] : make_uint2(0, 0);
         }
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
@@ -372,7 +383,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         // column counts, a wavefront per run (chunks wv, wv + NW, ...)
         for (int k = wv; k < nbin; k += NW) {
             const int s0 = row0[k], c = row1[k] - s0;
-            const uint4 *run = bins + ((long)k * BA + s0) * RW;
+            const uint2 *run = bins + ((long)k * BA + s0) * RW;
             for (int j = lane; j < c; j += 64)
                 atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
         }
@@ -406,9 +417,9 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + nb;
     }
     __syncthreads();
-    uint4 *dst = reinterpret_cast<uint4 *>(F.recs) + (long)pre * RW;
+    uint2 *dst = reinterpret_cast<uint2 *>(F.recs) + (long)pre * RW;
     if (staged) {
-        uint4 *sbuf = reinterpret_cast<uint4 *>(smem_raw);
+        uint2 *sbuf = reinterpret_cast<uint2 *>(smem_raw);
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
             if (q * NT + (int)threadIdx.x < nb) {
@@ -428,13 +439,13 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
                 const int i = i0 + q * NT + (int)threadIdx.x;
                 if (i < nb) {
                     const long g = rmap[i - i0];
-                    const uint4 v0 = bins[g * RW], v1 = bins[g * RW + 1];
-                    const uint4 v2 = RW > 2 ? bins[g * RW + 2] : v1;
+                    uint2 v[RW];
+#pragma unroll
+                    for (int w = 0; w < RW; ++w) v[w] = bins[g * RW + w];
                     // the cell by a load of its own (as below)
                     const long slot = atomicAdd(&cur[(rec_cell<T>(bins, (int)g) >> 10) & 1023], 1);
-                    dst[slot * RW] = v0;
-                    dst[slot * RW + 1] = v1;
-                    if (RW > 2) dst[slot * RW + 2] = v2;
+#pragma unroll
+                    for (int w = 0; w < RW; ++w) dst[slot * RW + w] = v[w];
                 }
             }
             __syncthreads();  // the map is rewritten for the next window
@@ -442,16 +453,16 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     } else {
         for (int k = wv; k < nbin; k += NW) {
             const int s0 = row0[k], c = row1[k] - s0;
-            const uint4 *run = bins + ((long)k * BA + s0) * RW;
+            const uint2 *run = bins + ((long)k * BA + s0) * RW;
             for (int j = lane; j < c; j += 64) {
-                const uint4 v0 = run[(long)j * RW], v1 = run[(long)j * RW + 1];
-                const uint4 v2 = RW > 2 ? run[(long)j * RW + 2] : v1;
+                uint2 v[RW];
+#pragma unroll
+                for (int w = 0; w < RW; ++w) v[w] = run[(long)j * RW + w];
                 // the cell by a load of its own: taken from the loaded words,
                 // the compiler moved the rest of them through scratch memory
                 const long slot = atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
-                dst[slot * RW] = v0;
-                dst[slot * RW + 1] = v1;
-                if (RW > 2) dst[slot * RW + 2] = v2;
+#pragma unroll
+                for (int w = 0; w < RW; ++w) dst[slot * RW + w] = v[w];
             }
         }
     }
