@@ -1014,6 +1014,77 @@ constexpr int kXRegThreads = 128;
 constexpr int kXRegN = 128;
 constexpr int kXRegC = 16;    // columns per workgroup
 constexpr int kXRegLS = 129;  // column stride of the exchange buffer (complex)
+
+// One image group of a point of the register x pass: the terms of xsep_point
+// for group gi (its images, multiplicity and sign sums), added when the image
+// is in the ball; returns mul f (0 outside the ball).
+__device__ __forceinline__ double xreg_group(XLine64 &L, const CellArgs &cl, double q00,
+                                             double max_m2, double fac, double cc, double el,
+                                             double k5, double gxe, double s, int nx, bool xny,
+                                             bool both, int kz, int gi) {
+    const int g = gi ? -1 : 1;
+    const int sy = L.yny ? g : 1, sz = L.yny ? 1 : g;
+    double m2;
+    if (!L.in_ball(cl, q00, max_m2, nx, sy, sz, kz, cc, 0.0, m2)) return 0.0;
+    const double r = recip_f64(m2);
+    const double f = gxe * el * r;
+    const double w = k5 * s * ((fac - r) * f);
+    const double nxd = nx;
+    const int mul = (xny ? 2 : 1) * (both ? 2 : 1);
+    const int ssy = (xny || both) ? 0 : nx * sy, ssz = (xny || both) ? 0 : nx * sz;
+    const double mw = (double)mul * w;
+    L.W0 += mw;
+    L.W2 = fma(mw * nxd, nxd, L.W2);
+    L.Wy = fma(w, (double)ssy, L.Wy);
+    L.Wz = fma(w, (double)ssz, L.Wz);
+    L.Wyz = gi ? L.Wyz - mw : L.Wyz + mw;
+    return (double)mul * f;
+}
+
+// The influence of the 16 points of a thread of the register x pass
+// (x-separable boxes): u[h][jj] holds X[e], e = 2 j + h + 16 jj.  The terms,
+// images and order of summation of xsep_point, written out point by point
+// (every value stays in its register, no loop); the second image group is
+// evaluated only in wavefronts that hold a Nyquist column.
+template <typename T>
+__device__ __forceinline__ void xsep_points_reg(XLine64 &L, const FrameDesc &F,
+                                                const double *gx, const double *k, double q00,
+                                                bool both, int ng, int kz, int j,
+                                                const double *ib0, cplx<T> (&u)[2][8]) {
+    const CellArgs &cl = F.tcell;
+    const double fac = F.ewald_fac, max_m2 = F.max_m2;
+    const bool two = __any(ng == 2);
+    // (the second group's constants are read where they are used: held for
+    // the whole loop, they cost the registers of the fourth workgroup per CU)
+    const double c0 = k[0], el = k[2], k4 = k[4], k5 = k[5];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int e = 2 * j + h + 16 * jj;
+            // nx = e for e <= 64, e - 128 above (K0 = 128)
+            const int nx = jj < 4 ? e : (jj > 4 ? e - kXRegN : (e == kXRegN / 2 ? e : e - kXRegN));
+            const bool xny = jj == 4 && e == kXRegN / 2;
+            const double ib = ib0[e] * k4;
+            cplx<T> &p = u[h][jj];
+            const double re = (double)p.x * ib, im = (double)p.y * ib;
+            const double s = re * re + im * im;
+            const double gxe = gx[nx < 0 ? -nx : nx];
+            double a = 0.0;
+            a = a + xreg_group(L, cl, q00, max_m2, fac, c0, el, k5, gxe, s, nx, xny, both, kz, 0);
+            if (two && ng == 2)
+                a = a + xreg_group(L, cl, q00, max_m2, fac, k[1], k[3], k5, gxe, s, nx, xny,
+                                   both, kz, 1);
+            L.aE = fma(a, s, L.aE);
+            const double tt = a * ib;
+            p.x = T(re * tt);
+            p.y = T(im * tt);
+            // one point at a time: the empty asm makes the point's results exist
+            // here (left to the compiler, their products sink to the end and
+            // every point's terms stay live at once)
+            asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(L.aE));
+        }
+}
 template <typename T> __host__ __device__ inline size_t xreg_lds_bytes() {
     return (size_t)kXRegN * sizeof(cplx<T>) + (size_t)kXRegN * sizeof(double) +
            (size_t)kXRegC * kXRegLS * sizeof(cplx<T>) +
@@ -1022,7 +1093,8 @@ template <typename T> __host__ __device__ inline size_t xreg_lds_bytes() {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kXRegThreads) fft_x_reg_kernel(SPME_FRAME_ARGS) {
+__global__ void __launch_bounds__(kXRegThreads) __attribute__((amdgpu_waves_per_eu(4)))
+fft_x_reg_kernel(SPME_FRAME_ARGS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int N = kXRegN, C = kXRegC, LS = kXRegLS;
     int b;
@@ -1083,20 +1155,7 @@ __global__ void __launch_bounds__(kXRegThreads) fft_x_reg_kernel(SPME_FRAME_ARGS
         const bool both = L.yny && L.zn2;
         const int ng = (L.yny || L.zn2) ? 2 : 1;
         const double *k = colk + kXColConsts * c;
-        if (valid) {
-            // one point per iteration from the head of the 16 values, put back
-            // at their tail (a rotation keeps every index static, so they stay
-            // in registers; after 16 iterations they are in order again)
-#pragma unroll 1
-            for (int i = 0; i < 16; ++i) {
-                const int e = 2 * j + (i >> 3) + 16 * (i & 7);
-                cplx<T> p = u[0][0];
-                xsep_point<T>(L, F, gx, k, q00, both, ng, kz, e, ib0[e], p);
-#pragma unroll
-                for (int q = 0; q < 15; ++q) u[q >> 3][q & 7] = u[(q + 1) >> 3][(q + 1) & 7];
-                u[1][7] = p;
-            }
-        }
+        if (valid) xsep_points_reg<T>(L, F, gx, k, q00, both, ng, kz, j, ib0, u);
         L.finish(cl, kz, wsums);
     }
     // inverse: 8-point inverse DFT over k2, twiddles, exchange (each thread
