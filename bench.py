@@ -65,10 +65,14 @@ def kernel_bytes(kernel, n, K, p):
     K0, K1, K2 = K
     G = K0 * K1 * K2
     H = K0 * K1 * (K2 // 2 + 1)
+    rec = 4 * p + 8  # atom record: fractions and charge in the working precision, cell, index
     table = {
-        "bin": n * 4 * 8 + n * (3 * p + 16),   # x, w in; records out (plane-ordered chunks)
-        "sort": 2 * n * (3 * p + 16),          # records in, records out in column order
-        "spread": n * (3 * p + 16) + G * p,    # column-sorted records in, grid out once
+        "bin": n * 4 * 8 + n * rec,            # x, w in; records out (plane-ordered chunks)
+        "sort": 2 * n * rec,                   # records in, records out in column order
+        "spread": n * rec + G * p,             # column-sorted records in, grid out once
+        # spreading fused with the forward y-z transform: records in (each read
+        # by the workgroups of its stencil's planes, from L2), half-complex out
+        "spread_fft_yz": n * rec + 2 * H * p,
         "fft_z_r2c": G * p + 2 * H * p,
         "fft_y_fwd": 4 * H * p,
         "fft_x_recip": 4 * H * p,              # closed-form influence: no table read
@@ -521,6 +525,7 @@ STAGE_OF_KERNEL = {
     "spme::bin_kernel": "bin",
     "spme::sort_kernel": "sort",
     "spme::spread_tile_kernel": "spread",
+    "spme::spread_slab_kernel": "spread_fft_yz",
     "spme::fft_yz_fwd_kernel": "fft_yz_fwd",
     "spme::fft_z_r2c_kernel": "fft_z_r2c",
     "spme::fft_y_kernel": "fft_y",
@@ -681,13 +686,16 @@ def main():
     # The spreading stage of the batched FP32 engine (bin + sort + spread, one
     # launch each), timed live with HIP events on the batch's stream, against
     # the spreading bytes of one evaluation: N 4p (x, w) + G p (grid written
-    # once), p = 4, for every frame of the launch.
+    # once), p = 4, for every frame of the launch.  Where the spreading runs
+    # fused with the forward y-z transform (spread_fft_yz), its grid leaves
+    # the launch as the half-complex plane: 2 H p in place of G p.
     B = args.batch
     ks = b32["kernels"]
-    stage = [k for k in ("bin", "sort", "spread") if k in ks]
+    stage = [k for k in ("bin", "sort", "spread", "spread_fft_yz") if k in ks]
     stage_us = sum(ks[k]["avg_us"] for k in stage)
     G = int(np.prod(sy.K))
-    Bk = (sy.n * 4 * 4 + G * 4) * B
+    H2 = 2 * sy.K[0] * sy.K[1] * (sy.K[2] // 2 + 1)
+    Bk = (sy.n * 4 * 4 + (H2 if "spread_fft_yz" in stage else G) * 4) * B
     achieved = Bk / (stage_us * 1e-6) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{sy.name}_fp32_batch{B}.json")

@@ -27,6 +27,7 @@
 #include "sfac.hpp"
 #include "slab_kernels.hpp"
 #include "spread_kernels.hpp"
+#include "spread_slab_kernels.hpp"
 #include "table_kernels.hpp"
 #include "correl_kernels.hpp"
 #include "real_kernels.hpp"
@@ -46,7 +47,8 @@ const char *stage_name(Stage s) {
         "bin",       "sort",      "spread",    "fft_z_r2c",
         "fft_y_fwd", "fft_x_fwd", "fft_x_recip", "influence", "energy",
         "fft_x_inv", "fft_y_inv", "fft_z_c2r", "gather",    "gather_sorted",
-        "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile", "fft_x_recip"};
+        "finish",    "fft_yz_fwd", "fft_yz_inv", "gather_tile", "fft_x_recip",
+        "spread_fft_yz"};
     return names[s];
 }
 
@@ -177,6 +179,10 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             }
             break;
         }
+        case ST_SPREAD_YZ:
+            spread_slab_kernel<T, ORDER><<<g, kSpreadSlabThreads, lds, st>>>(L.frames, L.offs,
+                                                                           L.nf, sp);
+            break;
         case ST_ZR2C: fft_z_r2c_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_YF: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
         case ST_XF_SCALE:
@@ -253,18 +259,18 @@ int finishes(Stage prev, Stage s) {
 }
 
 const Stage kSfacLines[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE};
-const Stage kSfacSlab[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_YZ_FWD, ST_XF_SCALE};
+const Stage kSfacSlab[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_SPREAD_YZ, ST_YZ_FWD, ST_XF_SCALE};
 const Stage kDe1Lines[] = {ST_INFL, ST_XI, ST_YI, ST_ZC2R};
 const Stage kDe1Slab[] = {ST_INFL, ST_XI, ST_YZ_INV};
 const Stage kEnStages[] = {ST_INFL_E, ST_FINISH};
 const Stage kEvalLines[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
                             ST_YF,   ST_XFUSED, ST_XREG, ST_YI, ST_ZC2R, ST_GATHER_SORTED};
-const Stage kEvalSlab[] = {ST_BIN, ST_SORT, ST_SPREAD,
-                           ST_YZ_FWD, ST_XFUSED, ST_XREG, ST_YZ_INV, ST_GATHER_SORTED};
+const Stage kEvalSlab[] = {ST_BIN,    ST_SORT,   ST_SPREAD, ST_SPREAD_YZ,    ST_YZ_FWD,
+                           ST_XFUSED, ST_XREG,   ST_YZ_INV, ST_GATHER_SORTED};
 const Stage kEvalLinesTile[] = {ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C,
                                 ST_YF,   ST_XFUSED, ST_XREG, ST_YI, ST_ZC2R, ST_GATHER_TILE};
-const Stage kEvalSlabTile[] = {ST_BIN, ST_SORT, ST_SPREAD,
-                               ST_YZ_FWD, ST_XFUSED, ST_XREG, ST_YZ_INV, ST_GATHER_TILE};
+const Stage kEvalSlabTile[] = {ST_BIN,    ST_SORT, ST_SPREAD, ST_SPREAD_YZ,  ST_YZ_FWD,
+                               ST_XFUSED, ST_XREG, ST_YZ_INV, ST_GATHER_TILE};
 
 template <size_t NS> StageSeq seq_of(const Stage (&a)[NS]) { return StageSeq{a, (int)NS}; }
 
@@ -282,12 +288,16 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        const void *fns[6] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
-                              (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::medium>,
-                              (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::large>,
-                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::small>,
-                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::medium>,
-                              (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::large>};
+        const void *fns[10] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
+                               (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::medium>,
+                               (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::large>,
+                               (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::small>,
+                               (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::medium>,
+                               (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::large>,
+                               (const void *)spread_slab_kernel<T, 4>,
+                               (const void *)spread_slab_kernel<T, 6>,
+                               (const void *)spread_slab_kernel<T, 8>,
+                               (const void *)spread_slab_kernel<T, 0>};
         ok = true;
         for (const void *fn : fns)
             ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -492,6 +502,10 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                   slab_ept() > 0 &&
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
+        spread_slab_ok =
+            slab_ok && fixed &&
+            (fp32 ? spread_slab_fits<float>(K[0], K[1], K[2], ldim, order)
+                  : spread_slab_fits<double>(K[0], K[1], K[2], ldim, order));
         // the larger slab variants have no radix-16 stage
         if (use_slab && slab_ept() > 16)
             for (int d = 1; d < 3; ++d) {
@@ -811,7 +825,9 @@ int Engine::stage_blocks(Stage s) const { return stage_blocks_n(s, desc_.n); }
 
 bool Engine::uses_stage(Stage s) const {
     switch (s) {
-        case ST_YZ_FWD:
+        case ST_SPREAD: return !spread_slab();
+        case ST_SPREAD_YZ: return spread_slab();
+        case ST_YZ_FWD: return slab() && !spread_slab();
         case ST_YZ_INV: return slab();
         case ST_ZR2C:
         case ST_YF:
@@ -831,7 +847,8 @@ int Engine::stage_blocks_n(Stage s, int natoms) const {
     switch (s) {
         case ST_BIN: return bin_count(natoms, fp32);
         case ST_SORT: return K[0];
-        case ST_SPREAD: return D.ntx * D.nty * D.ntz;
+        case ST_SPREAD: return spread_slab() ? 0 : D.ntx * D.nty * D.ntz;
+        case ST_SPREAD_YZ: return spread_slab() ? K[0] : 0;
         case ST_ZR2C:
         case ST_ZC2R: return (nrows + 2 * D.zC - 1) / (2 * D.zC);
         case ST_YF:
@@ -840,7 +857,7 @@ int Engine::stage_blocks_n(Stage s, int natoms) const {
         case ST_XREG: return xreg() ? D.xB : 0;
         case ST_XF_SCALE:
         case ST_XI: return D.xB;
-        case ST_YZ_FWD:
+        case ST_YZ_FWD: return spread_slab() ? 0 : K[0];
         case ST_YZ_INV: return K[0];
         case ST_INFL:
         case ST_INFL_E: return nred_infl_;
@@ -866,6 +883,9 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
         case ST_SPREAD:  // chunked tiles: the compacted atom list after the cells
             return (size_t)D.tx * D.ty * (D.ntz > 1 ? D.tz : K[2] + order - 1) * 8 +
                    (D.ntz > 1 ? kSpreadBlock * sizeof(int) : 0);
+        case ST_SPREAD_YZ:
+            return fp32 ? spread_slab_lds_bytes<float>(K[1], K[2], order)
+                        : spread_slab_lds_bytes<double>(K[1], K[2], order);
         case ST_GATHER_TILE:  // chunked tiles: the compacted atom list after the rows
             return gather_tile_lds(D.gtx, D.gty, D.gntz > 1 ? D.gtz : K[2], order,
                                    fp32 ? 4 : 8) +
@@ -970,7 +990,7 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
     const void *key[6] = {w_d, x_d, out_d, dx_d, phi_d, desc_.recs};
     const bool graph = use_graph && !prof_;
     if (graph) {
-        const bool same = graph_exec_ && graph_n_ == n && graph_slab_ == slab() &&
+        const bool same = graph_exec_ && graph_n_ == n && graph_variant_ == variant() &&
                           memcmp(key, graph_key_, sizeof(key)) == 0;
         // the descriptor upload stays outside the graph, and runs before every
         // replay: the split entry points (sfac/en/de1/de2/potl) upload their
@@ -990,7 +1010,7 @@ void Engine::evaluate(int n, const double *w_d, const double *x_d, double *out_d
             HIPC(hipStreamEndCapture(stream_, &graph_));
             HIPC(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
             graph_n_ = n;
-            graph_slab_ = slab();
+            graph_variant_ = variant();
             memcpy(graph_key_, key, sizeof(key));
         }
         HIPC(hipGraphLaunch(graph_exec_, stream_));
@@ -1202,6 +1222,7 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
     v |= (unsigned long long)(x_influence_form(desc_) & 3) << 50;
     v |= (unsigned long long)(xreg() ? 1 : 0) << 52;
+    v |= (unsigned long long)(spread_slab() ? 1 : 0) << 53;
     return v;
 }
 
@@ -1225,7 +1246,7 @@ TuneCfg Engine::tune_cfg() const {
     c.gtx = desc_.gtx;
     c.gty = desc_.gty;
     c.gtz = desc_.gntz > 1 ? desc_.gtz : 0;
-    c.slab = use_slab ? 1 : 0;
+    c.slab = !use_slab ? 0 : use_spread_slab ? 2 : 1;
     return c;
 }
 
@@ -1314,6 +1335,16 @@ void Engine::apply_tune(const TuneCfg &c) {
     D.gtz = c.gtz ? c.gtz : K[2];
     D.gntz = (K[2] + D.gtz - 1) / D.gtz;
     use_slab = c.slab != 0;
+    use_spread_slab = c.slab != 1;
+    ++gen_;
+    drop_graph_();
+}
+
+void Engine::set_transform(int mode) {
+    if (mode < 0 || mode > 2) throw std::invalid_argument("transform mode must be 0, 1 or 2");
+    use_slab = mode != 1;
+    use_spread_slab = mode == 0;
+    // a batch holding this frame takes its launches anew
     ++gen_;
     drop_graph_();
 }
@@ -1851,8 +1882,16 @@ void Batch::refresh_variant_() {
         }
         xpath_ = std::max(xpath_, x_path(e->desc()));  // the most general path of the frames
     }
-    seq_.assign({ST_BIN, ST_SORT, ST_SPREAD});
-    if (any_slab) seq_.push_back(ST_YZ_FWD);
+    bool any_tiles = false, any_fused = false, any_yzf = false;
+    for (const Engine *e : frames_) {
+        any_tiles = any_tiles || !e->spread_slab();
+        any_fused = any_fused || e->spread_slab();
+        any_yzf = any_yzf || (e->slab() && !e->spread_slab());
+    }
+    seq_.assign({ST_BIN, ST_SORT});
+    if (any_tiles) seq_.push_back(ST_SPREAD);
+    if (any_fused) seq_.push_back(ST_SPREAD_YZ);
+    if (any_yzf) seq_.push_back(ST_YZ_FWD);
     if (any_lines) seq_.insert(seq_.end(), {ST_ZR2C, ST_YF});
     bool any_xreg = false, any_xfused = false;
     for (const Engine *e : frames_) {

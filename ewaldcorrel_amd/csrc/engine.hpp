@@ -32,7 +32,7 @@ enum EngineFlags {
 enum Stage {
     ST_BIN, ST_SORT, ST_SPREAD, ST_ZR2C, ST_YF, ST_XF_SCALE, ST_XFUSED,
     ST_INFL, ST_INFL_E, ST_XI, ST_YI, ST_ZC2R, ST_GATHER, ST_GATHER_SORTED, ST_FINISH,
-    ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_XREG, ST_COUNT
+    ST_YZ_FWD, ST_YZ_INV, ST_GATHER_TILE, ST_XREG, ST_SPREAD_YZ, ST_COUNT
 };
 
 // stage sequences of the entry points for the slab / line transforms
@@ -201,6 +201,15 @@ public:
     bool slab_ok = false;
     bool use_slab = false;
     bool slab() const { return slab_ok && use_slab; }
+    // spreading fused with the forward slab transform (one workgroup per x
+    // plane, the plane's sums in LDS) where that applies; use_spread_slab =
+    // false keeps the separate spreading tiles before the slab transform
+    bool spread_slab_ok = false;
+    bool use_spread_slab = true;
+    bool spread_slab() const { return slab() && spread_slab_ok && use_spread_slab; }
+    // 0: slab transforms (spreading fused where it applies), 1: line passes,
+    // 2: slab transforms after spreading tiles; std::invalid_argument otherwise
+    void set_transform(int mode);
     int slab_ept() const;
     // the evaluation gathers through LDS-staged tiles when one fits (FrameDesc::gtx)
     bool gtile() const { return desc_.gtx > 0; }
@@ -265,7 +274,7 @@ private:
     void free_(void *p);  // release one alloc_ buffer (stream synchronised)
     void run_(Stage s, int finish);  // single-frame launch of one stage on stream_
     void run_seq_(StageSeq seq);
-    bool graph_slab_ = false;        // path the captured graph was recorded with
+    unsigned long long graph_variant_ = 0;  // kernels the captured graph was recorded with
     void drop_graph_();
     void set_atoms_(int n, const double *w, const double *x, double *dx, double *phi,
                     double *out);

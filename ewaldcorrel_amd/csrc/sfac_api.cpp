@@ -353,8 +353,7 @@ int sfac_sync(void *h) {
 
 int sfac_set_transform(void *h, int mode) {
     return guard([&] {
-        if (mode != 0 && mode != 1) throw std::invalid_argument("transform mode must be 0 or 1");
-        as_sfac(h)->engine->use_slab = mode == 0;
+        as_sfac(h)->engine->set_transform(mode);
     });
 }
 
@@ -372,7 +371,7 @@ int sfac_info(void *h, long long out[12]) {
         out[7] = e->state;
         out[8] = e->table_mode();
         out[9] = e->use_graph ? 1 : 0;
-        out[10] = e->slab() ? 1 : 0;
+        out[10] = !e->slab() ? 0 : e->spread_slab() ? 2 : 1;
         out[11] = e->slab_ok ? 1 : 0;
     });
 }

@@ -105,8 +105,14 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
         // transform path
         if (e0->slab_ok) {
             TuneCfg c = cur;
-            c.slab = 1 - cur.slab;
+            c.slab = cur.slab ? 0 : 2;
             trial(c, std::string("transforms ") + (c.slab ? "slab" : "lines"));
+        }
+        // spreading fused with the forward slab transform, or in tiles before it
+        if (e0->slab_ok && e0->spread_slab_ok && cur.slab != 0) {
+            TuneCfg c = cur;
+            c.slab = cur.slab == 2 ? 1 : 2;
+            trial(c, std::string("spreading ") + (c.slab == 2 ? "fused" : "in tiles"));
         }
         // radix order of each axis (z, y, x: the slab passes first)
         for (int d : {2, 1, 0}) {
@@ -122,8 +128,10 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
             return std::to_string(t[0]) + "x" + std::to_string(t[1]) +
                    (t[2] ? "x" + std::to_string(t[2]) : std::string());
         };
-        // spreading tile (z chunks of tz cells, or whole columns)
+        // spreading tile (z chunks of tz cells, or whole columns; none when the
+        // spreading runs fused with the slab transform)
         for (const auto &t : e0->spread_tile_options()) {
+            if (e0->spread_slab()) break;
             if (t[0] == cur.tx && t[1] == cur.ty && t[2] == cur.tz) continue;
             TuneCfg c = cur;
             c.tx = t[0];

@@ -403,7 +403,11 @@ class Sfac:
         return buf.value.decode()
 
     def set_transform(self, mode):
-        """0: y-z slab transforms where available (default); 1: z and y line passes."""
+        """0: y-z slab transforms where available, the spreading fused into the
+        forward one where its plane of sums fits in LDS (default); 1: z and y
+        line passes; 2: y-z slab transforms after separate spreading tiles.
+        info()["slab"] reports the path: 0 lines, 1 slab, 2 slab with fused
+        spreading."""
         _check(self._libsfac.sfac_set_transform(self._data, int(mode)), "set_transform")
 
     def info(self):

@@ -83,12 +83,17 @@ int sfac_sync(void *h);
 
 /* out[0..11] = K0, K1, K2, ldim, order, flags, device bytes held, state,
  * influence-table mode (0 table in HBM, 1 closed-form Ewald from the
- * indices), graph replay enabled, y-z slab transforms in use, slab
- * transforms available for this grid */
+ * indices), graph replay enabled, y-z slab transforms in use (0 line
+ * passes, 1 slab transforms after spreading tiles, 2 slab transforms with
+ * the spreading fused into the forward one), slab transforms available for
+ * this grid */
 int sfac_info(void *h, long long out[12]);
 
 /* Transform path: 0 (default) the y-z slab transforms when the plane fits
- * in LDS and K1, K2 factor into 2, 3, 5; 1 separate z and y line passes. */
+ * in LDS and K1, K2 factor into 2, 3, 5, with the spreading fused into the
+ * forward slab transform where a plane of fixed-point sums fits in LDS as
+ * well; 1 separate z and y line passes; 2 the slab transforms after separate
+ * spreading tiles. */
 int sfac_set_transform(void *h, int mode);
 
 /* sfac_eval_dev captures its launches in a hipGraph and replays it while

@@ -64,13 +64,13 @@ def test_slab_equals_line_passes(grid, precision, fixed):
     s.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
     info = s.info()
     say(name, precision, fixed, "slab", info["slab"], "available", info["slab_available"])
-    assert info["slab_available"] == 1 and info["slab"] == 1
+    assert info["slab_available"] == 1 and info["slab"] >= 1
     a = evaluate(s, sy)
     s.set_transform(1)
     assert s.info()["slab"] == 0
     b = evaluate(s, sy)
     s.set_transform(0)
-    assert s.info()["slab"] == 1
+    assert s.info()["slab"] == info["slab"]
     c = evaluate(s, sy)
     tol = 1e-13 if precision == "fp64" else 1e-6
     say("  E", abs(a[0] - b[0]) / abs(b[0]), "vir", rel(a[1], b[1]), "F", rel_rms(a[2], b[2]),
@@ -94,7 +94,7 @@ def test_fp32_largest_slab_variant():
     sy = make_sys([10., 40., 60., 0.5, -1., 2.], (8, 128, 192), 4, 2500, 7)
     s = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision="fp32")
     s.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
-    assert s.info()["slab"] == 1
+    assert s.info()["slab"] >= 1
     a = evaluate(s, sy)
     s.set_transform(1)
     b = evaluate(s, sy)
@@ -149,4 +149,4 @@ def test_slab_availability(precision, K, available):
     info = s.info()
     say(precision, K, info)
     assert info["slab_available"] == available
-    assert info["slab"] == available
+    assert (info["slab"] >= 1) == (available == 1)
