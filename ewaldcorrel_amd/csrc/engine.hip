@@ -26,6 +26,7 @@
 #include "recip_kernels.hpp"
 #include "sfac.hpp"
 #include "slab_kernels.hpp"
+#include "slab128_kernels.hpp"
 #include "spread_kernels.hpp"
 #include "spread_slab_kernels.hpp"
 #include "table_kernels.hpp"
@@ -180,8 +181,12 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             break;
         }
         case ST_SPREAD_YZ:
-            spread_slab_kernel<T, ORDER><<<g, kSpreadSlabThreads, lds, st>>>(L.frames, L.offs,
-                                                                           L.nf, sp);
+            if (std::is_same<T, float>::value && L.slab_ept == kSlabKernel128)
+                spread_slab_kernel<T, ORDER, true><<<g, kSpreadSlabThreads, lds, st>>>(
+                    L.frames, L.offs, L.nf, sp);
+            else
+                spread_slab_kernel<T, ORDER, false><<<g, kSpreadSlabThreads, lds, st>>>(
+                    L.frames, L.offs, L.nf, sp);
             break;
         case ST_ZR2C: fft_z_r2c_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf); break;
         case ST_YF: fft_y_kernel<T><<<g, bl, lds, st>>>(L.frames, L.offs, L.nf, -1, 0); break;
@@ -197,6 +202,17 @@ void launch_t(Stage s, const Launch &L, int blocks, size_t lds,
             break;
         case ST_YZ_FWD:
         case ST_YZ_INV: {
+            if constexpr (std::is_same<T, float>::value) {
+                if (L.slab_ept == kSlabKernel128) {
+                    if (s == ST_YZ_FWD)
+                        fft_yz_fwd128_kernel<<<g, kSlab128Threads, lds, st>>>(L.frames, L.offs,
+                                                                              L.nf);
+                    else
+                        fft_yz_inv128_kernel<<<g, kSlab128Threads, lds, st>>>(L.frames, L.offs,
+                                                                              L.nf, finish);
+                    break;
+                }
+            }
             auto go = [&](auto ept_c) {
                 constexpr int E = decltype(ept_c)::value;
                 if (s == ST_YZ_FWD)
@@ -288,20 +304,29 @@ template <typename T> bool slab_lds_usable(size_t bytes) {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        const void *fns[10] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
+        const void *fns[14] = {(const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::small>,
                                (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::medium>,
                                (const void *)fft_yz_fwd_kernel<T, SlabCfg<T>::large>,
                                (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::small>,
                                (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::medium>,
                                (const void *)fft_yz_inv_kernel<T, SlabCfg<T>::large>,
-                               (const void *)spread_slab_kernel<T, 4>,
-                               (const void *)spread_slab_kernel<T, 6>,
-                               (const void *)spread_slab_kernel<T, 8>,
-                               (const void *)spread_slab_kernel<T, 0>};
+                               (const void *)spread_slab_kernel<T, 4, false>,
+                               (const void *)spread_slab_kernel<T, 6, false>,
+                               (const void *)spread_slab_kernel<T, 8, false>,
+                               (const void *)spread_slab_kernel<T, 0, false>,
+                               (const void *)spread_slab_kernel<T, 4, true>,
+                               (const void *)spread_slab_kernel<T, 6, true>,
+                               (const void *)spread_slab_kernel<T, 8, true>,
+                               (const void *)spread_slab_kernel<T, 0, true>};
         ok = true;
         for (const void *fn : fns)
             ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)kSlabLds) == hipSuccess;
+        if (std::is_same<T, float>::value)
+            for (const void *fn : {(const void *)fft_yz_fwd128_kernel,
+                                   (const void *)fft_yz_inv128_kernel})
+                ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)kSlabLds) == hipSuccess;
         if (!ok) (void)hipGetLastError();
     });
     return ok;
@@ -502,6 +527,7 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
                   slab_ept() > 0 &&
                   (fp32 ? slab_lds_usable<float>(lds) : slab_lds_usable<double>(lds));
         use_slab = slab_ok;
+        slab128_ok_ = slab_ok && fp32 && slab128_lds_bytes() <= kSlabLds;
         spread_slab_ok =
             slab_ok && fixed &&
             (fp32 ? spread_slab_fits<float>(K[0], K[1], K[2], ldim, order)
@@ -906,6 +932,7 @@ size_t Engine::stage_lds(Stage s, int xpath) const {
                    xfused_extra_lds(K[0], D.xC);
         case ST_YZ_FWD:
         case ST_YZ_INV:
+            if (slab_kernel() == kSlabKernel128) return slab128_lds_bytes();
             return fp32 ? slab_lds_bytes<float>(K[1], K[2], ldim)
                         : slab_lds_bytes<double>(K[1], K[2], ldim);
         default: return 0;
@@ -917,6 +944,12 @@ void Engine::launch(Stage s, const Launch &L, int blocks, size_t lds,
     if (blocks <= 0) return;
     if (fp32) launch_order<float>(order, s, L, blocks, lds, st, sp_f_, fixed, finish);
     else launch_order<double>(order, s, L, blocks, lds, st, sp_d_, fixed, finish);
+}
+
+int Engine::slab_kernel() const {
+    if (fp32 && slab() && slab128_ok_ && slab128_applies(K[1], K[2], desc_.plan1, desc_.plan2))
+        return kSlabKernel128;
+    return slab_ept();
 }
 
 int Engine::slab_ept() const {
@@ -935,7 +968,7 @@ void Engine::run_(Stage s, int finish) {
     if (nb <= 0) return;
     sync_desc_();
     ProfScope ps(this, stage_name(s));
-    launch(s, Launch{d_desc_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
+    launch(s, Launch{d_desc_, nullptr, 1, slab_kernel(), x_path(desc_), gmz(), desc_.ntz > 1,
                      desc_.gntz > 1, x_influence_form(desc_)}, nb,
            stage_lds(s), stream_,
            finish);
@@ -1074,7 +1107,7 @@ void Engine::evaluate_noalloc(int n, const double *w_d, const double *x_d, doubl
     // pageable source: the call returns once the bytes are staged
     HIPC(hipMemcpyAsync(d_desc_na_, &desc_na_, sizeof(FrameDesc), hipMemcpyHostToDevice,
                         stream_));
-    const Launch L{d_desc_na_, nullptr, 1, slab_ept(), x_path(desc_), gmz(), desc_.ntz > 1,
+    const Launch L{d_desc_na_, nullptr, 1, slab_kernel(), x_path(desc_), gmz(), desc_.ntz > 1,
                    desc_.gntz > 1, x_influence_form(desc_)};
     const StageSeq seq = eval_stages(slab(), gtile());
     Stage prev = ST_COUNT;
@@ -1216,7 +1249,7 @@ unsigned long long Engine::variant() const {
     v |= (unsigned long long)(gtile() ? 1 : 0) << 3;
     v |= (unsigned long long)(x_path(desc_) & 0xf) << 4;
     v |= (unsigned long long)(order & 0xff) << 8;
-    v |= (unsigned long long)(slab_ept() & 0xffff) << 16;
+    v |= (unsigned long long)(slab_kernel() & 0xffff) << 16;
     v |= (unsigned long long)(gmz() & 0xffff) << 32;
     v |= (unsigned long long)(desc_.ntz > 1 ? 1 : 0) << 48;
     v |= (unsigned long long)(desc_.gntz > 1 ? 1 : 0) << 49;
@@ -1870,7 +1903,7 @@ void Batch::refresh_variant_() {
         gzchunk_ = gzchunk_ || e->desc().gntz > 1;
         if (e->slab()) {
             any_slab = true;
-            slab_ept_ = std::max(slab_ept_, e->slab_ept());
+            slab_ept_ = std::max(slab_ept_, e->slab_kernel());
         } else {
             any_lines = true;
         }

@@ -205,12 +205,18 @@ public:
     // plane, the plane's sums in LDS) where that applies; use_spread_slab =
     // false keeps the separate spreading tiles before the slab transform
     bool spread_slab_ok = false;
+    bool slab128_ok_ = false;  // the register-line slab kernels fit this device's LDS
     bool use_spread_slab = true;
     bool spread_slab() const { return slab() && spread_slab_ok && use_spread_slab; }
     // 0: slab transforms (spreading fused where it applies), 1: line passes,
     // 2: slab transforms after spreading tiles; std::invalid_argument otherwise
     void set_transform(int mode);
     int slab_ept() const;
+    // the slab kernels a launch runs: kSlabKernel128 for the register-line
+    // variant of 128 x 128 FP32 planes with the plans (16, 8), else slab_ept()
+    // (a batch runs the largest code of its frames: the generic kernels
+    // take every plane)
+    int slab_kernel() const;
     // the evaluation gathers through LDS-staged tiles when one fits (FrameDesc::gtx)
     bool gtile() const { return desc_.gtx > 0; }
     int gmz() const {

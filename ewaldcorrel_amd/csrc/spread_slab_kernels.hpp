@@ -11,6 +11,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "slab128_kernels.hpp"
 #include "slab_kernels.hpp"
 #include "spread_kernels.hpp"
 
@@ -44,7 +45,10 @@ __host__ inline bool spread_slab_fits(int K0, int K1, int K2, int ldim, int orde
            std::max(npair * K2, ldim * K1) <= NT * EPT && npair * ldim <= NT * (EPT / 2 + 1);
 }
 
-template <typename T, int ORDER>
+// R128: FP32 128 x 128 planes with the plans (16, 8): the transform of the
+// register-line slab kernels (slab128_fwd_lines) in place of the LDS-staged
+// line passes, the fold writing the z lines straight into registers.
+template <typename T, int ORDER, bool R128>
 __global__ void __launch_bounds__(kSpreadSlabThreads)
 spread_slab_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -175,23 +179,39 @@ spread_slab_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
             if (dq[q] >= 0) atom(v[q], dq[q]);
     }
     __syncthreads();
+    const double inv = 1.0 / scale;
+    auto cellv = [&](int y, int z) -> T {
+        int zz = z + H;
+        if (zz >= K2) zz -= K2;
+        const unsigned long long *arow = acc + y * Kp;
+        unsigned long long v = arow[zz];
+        if (zz < n - 1) v += arow[zz + K2];
+        return T((double)(long long)v * inv);
+    };
+    if constexpr (R128 && std::is_same<T, float>::value) {
+        // z line c = t / 8 (rows 2c, 2c + 1), j = t mod 8: x[j + 8 r] folded
+        // and converted into registers, then the register-line transform
+        constexpr int N = kSlab128N;
+        float2 v[16];
+        const int t = threadIdx.x;
+        const int la = t < 8 * (N / 2) ? t >> 3 : 0, ja = t & 7;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            v[r].x = cellv(2 * la, ja + 8 * r);
+            v[r].y = cellv(2 * la + 1, ja + 8 * r);
+        }
+        __syncthreads();
+        slab128_fwd_lines<NT>(F, x0, reinterpret_cast<float2 *>(acc), twz, twy, v);
+        return;
+    }
     // fold the z halo (cell z holds zz = (z + H) mod K2, and zz + K2 for
     // zz < n - 1) and convert into the pair layout: element (c, z) =
     // row_2c[z] + i row_2c+1[z] at slab[c 2 RS + z]; all reads first, as the
     // slab overlays the sums
     {
-        const double inv = 1.0 / scale;
         constexpr int Q = EPT;  // npair K2 <= NT EPT elements
         const int tot = npair * K2;
         T va[Q], vb[Q];
-        auto cellv = [&](int y, int z) -> T {
-            int zz = z + H;
-            if (zz >= K2) zz -= K2;
-            const unsigned long long *arow = acc + y * Kp;
-            unsigned long long v = arow[zz];
-            if (zz < n - 1) v += arow[zz + K2];
-            return T((double)(long long)v * inv);
-        };
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int l = (int)threadIdx.x + q * NT;

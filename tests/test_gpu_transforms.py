@@ -48,6 +48,8 @@ GRIDS = [
     ("many_x_blocks", [40., 44., 46., 2., 1., -3.], (48, 96, 96), 4, 3000),
     # 96 x 97 = 9312 plane elements: the slab variant for large planes
     ("large_slab_variant", [15., 40., 60., 1., -2., 3.], (16, 96, 192), 4, 1500),
+    # 128 x 128 planes: FP32 runs the register-line slab kernels
+    ("planes_128", [30., 34., 32., 2., 0., -1.], (24, 128, 128), 4, 3000),
 ]
 
 
