@@ -63,6 +63,7 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
     Timer time(b, frames, st, reps);
     std::string rep;
     char line[512];
+    (void)time();  // clocks and caches warm before the first measurement
     double best = time();
     const double t_start = best;
     snprintf(line, sizeof(line), "start %.4f ms per batch evaluation (%d frames)\n", best,
@@ -82,7 +83,10 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
         Engine *e0 = cls[0];
         TuneCfg cur = e0->tune_cfg();
         for (Engine *e : cls) e->apply_tune(cur);  // one shape for the whole class
-        // try one candidate: keep it if it beats the best by 1 %
+        // try one candidate: keep it if it beats the best by 1 %, confirmed
+        // by measuring the current shapes and the candidate once more (the
+        // candidate's slower time against the current shapes' faster one), so
+        // that run-to-run noise does not pick a shape
         auto trial = [&](const TuneCfg &c, const std::string &what) {
             try {
                 for (Engine *e : cls) e->apply_tune(c);
@@ -90,8 +94,15 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
                 for (Engine *e : cls) e->apply_tune(cur);
                 return;
             }
-            const double t = time();
-            const bool keep = t < 0.99 * best;
+            double t = time();
+            bool keep = t < 0.99 * best;
+            if (keep) {
+                for (Engine *e : cls) e->apply_tune(cur);
+                best = std::min(best, time());
+                for (Engine *e : cls) e->apply_tune(c);
+                t = std::max(t, time());
+                keep = t < 0.99 * best;
+            }
             snprintf(line, sizeof(line), "%s %s: %.4f ms%s\n", key.c_str(), what.c_str(), t,
                      keep ? " (kept)" : "");
             rep += line;
