@@ -527,6 +527,8 @@ STAGE_OF_KERNEL = {
     "spme::spread_tile_kernel": "spread",
     "spme::spread_slab_kernel": "spread_fft_yz",
     "spme::fft_yz_fwd_kernel": "fft_yz_fwd",
+    "spme::fft_yz_fwd128_kernel": "fft_yz_fwd",
+    "spme::fft_yz_inv128_kernel": "fft_yz_inv",
     "spme::fft_z_r2c_kernel": "fft_z_r2c",
     "spme::fft_y_kernel": "fft_y",
     "spme::fft_x_kernel<float, 2": "fft_x_recip",

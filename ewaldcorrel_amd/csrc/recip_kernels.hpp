@@ -1110,10 +1110,20 @@ fft_x_reg_kernel(SPME_FRAME_ARGS) {
     const int f = b * C + c;
     const bool valid = f < P;
     const int fc = valid ? f : P - 1;  // clamped for loads
-    cplx<T> *col = (cplx<T> *)F.grid + fc;
+    // element x of the column at 32-bit offset fc + x P (complex) from the
+    // frame's grid, a workgroup-uniform global base: one register per
+    // address, recomputed for the stores (16 64-bit addresses kept from the
+    // loads to the stores were spilled)
+    typedef __attribute__((address_space(1))) T GT;
+    GT *grid = (GT *)F.grid;
+    const unsigned e0 = 2u * ((unsigned)fc + (unsigned)j * (unsigned)P);
+    const unsigned es = 16u * (unsigned)P;
     cplx<T> v[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = col[(long)(j + 8 * r) * P];
+    for (int r = 0; r < 16; ++r) {
+        v[r].x = grid[e0 + (unsigned)r * es];
+        v[r].y = grid[e0 + (unsigned)r * es + 1u];
+    }
     {
         const cplx<T> *tw = (const cplx<T> *)F.tw0;
         for (int e = t; e < N; e += kXRegThreads) {
@@ -1173,9 +1183,14 @@ fft_x_reg_kernel(SPME_FRAME_ARGS) {
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) v[k1] = buf[c * LS + k1 * 8 + j];
     bfly16<T>(v, +1);
+    unsigned e1 = e0;
+    asm volatile("" : "+v"(e1));
     if (valid) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) col[(long)(j + 8 * r) * P] = v[r];
+        for (int r = 0; r < 16; ++r) {
+            grid[e1 + (unsigned)r * es] = v[r].x;
+            grid[e1 + (unsigned)r * es + 1u] = v[r].y;
+        }
     }
     // the workgroup's partial: its wavefront sums in order (made visible by
     // the barrier above)
