@@ -371,6 +371,19 @@ template <typename T, int ORDER> bool spread_attr() {
                                  (int)kTileLdsMax) == hipSuccess;
     return ok;
 }
+// the bin workgroup's records in LDS (72 KB for FP32): above the default
+// 64 KB of dynamic LDS, set once per precision
+template <typename T> bool bin_attr_ok() {
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        ok = hipFuncSetAttribute((const void *)bin_kernel<T>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)bin_lds_bytes<T>()) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+    });
+    return ok;
+}
 template <typename T> bool tile_attrs_ok() {
     static std::once_flag once;
     static bool ok = false;
@@ -442,6 +455,11 @@ Engine::Engine(const double L6_[6], const int K_[3], int order_, int flags) {
     desc_.tcell = desc_.cell;
 
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!(fp32 ? bin_attr_ok<float>() : bin_attr_ok<double>()))
+        throw std::runtime_error("the bin kernel's LDS (" +
+                                 std::to_string(fp32 ? bin_lds_bytes<float>()
+                                                     : bin_lds_bytes<double>()) +
+                                 " bytes) is not available on this device");
     const size_t tsz = fp32 ? sizeof(float) : sizeof(double);
     alloc_(&grid_, grid_reals() * tsz);
     for (int d = 0; d < 3; ++d) {

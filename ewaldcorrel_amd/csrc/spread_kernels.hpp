@@ -137,7 +137,10 @@ __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx,
 // atomics and no capacity limit: a plane may hold any number of atoms.
 constexpr int kBinThreads = 1024;
 template <typename T> __host__ __device__ constexpr int bin_atoms() {
-    return std::is_same<T, float>::value ? 2048 : 1024;  // 64 KB / 48 KB of records in LDS
+    // 72 KB (FP32) / 48 KB (FP64) of records in LDS; 3072 FP32 atoms per
+    // workgroup: a 16-frame water launch (512 workgroups) in one round of
+    // two per CU (2048 measured 1.8 us slower: a second, half-empty round)
+    return std::is_same<T, float>::value ? 3072 : 1024;
 }
 template <typename T> __host__ __device__ constexpr size_t bin_lds_bytes() {
     return (size_t)bin_atoms<T>() * sizeof(AtomRec<T>);
