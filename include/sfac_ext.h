@@ -63,9 +63,9 @@ int sfac_eval_dev(void *h, int n, const double *w_d, const double *x_d, double *
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc); no counterpart in the reference.  The results are those of
  * sfac_eval_dev.  The per-atom sort workspace lives in the library's static
- * scratch array of SFAC_NOALLOC_SCRATCH_BYTES per GPU (64 B per atom in FP32,
- * 96 B in FP64, plus K0 + 1 ints per 2,048 FP32 or 1,024 FP64 atoms: about
- * 1.0 M atoms in FP32 or 0.69 M in FP64 on a 128-point x axis), shared by
+ * scratch array of SFAC_NOALLOC_SCRATCH_BYTES per GPU (48 B per atom in FP32,
+ * 96 B in FP64, plus K0 + 1 ints per 3,072 FP32 or 1,024 FP64 atoms: about
+ * 1.39 M atoms in FP32 or 0.69 M in FP64 on a 128-point x axis), shared by
  * every handle on that GPU:
  * calls that overlap on one GPU race.  Kernels are launched one by one (no
  * graph capture or instantiation).  Returns SFAC_ERR_ARG, with the limit in
