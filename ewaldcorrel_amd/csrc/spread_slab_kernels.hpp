@@ -67,25 +67,39 @@ spread_slab_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
     cplx<T> *twy = twz + K2;
     unsigned long long *acc = reinterpret_cast<unsigned long long *>(twy + K1);
     cplx<T> *slab = reinterpret_cast<cplx<T> *>(acc);
+    // segment d: the records of plane x0 - n/2 + d (mod K0), whose atoms reach
+    // plane x0 through their x weight d (lane d of the first wavefront; the
+    // loads in flight while LDS is cleared)
+    int sbeg = 0, slen = 0;
+    if ((int)threadIdx.x < n) {
+        const int cx = imod(x0 - n / 2 + (int)threadIdx.x, K0);
+        sbeg = F.colstart[cx * K1];
+        slen = F.colstart[(cx + 1) * K1] - sbeg;
+    }
+    // the twiddles to registers now, to LDS after the atoms (their region is
+    // not touched before): no wait for them at the start
+    const cplx<T> *gtz = (const cplx<T> *)F.tw2, *gty = (const cplx<T> *)F.tw1;
+    cplx<T> rtz{}, rty{};
+    if ((int)threadIdx.x < K2) rtz = gtz[threadIdx.x];
+    if ((int)threadIdx.x < K1) rty = gty[threadIdx.x];
     {
         const int cells = K1 * Kp;
         uint4 *z4 = reinterpret_cast<uint4 *>(acc);  // (K1 + K2) * 8 B in: 16-B aligned
         for (int i = threadIdx.x; i < cells / 2; i += NT) z4[i] = make_uint4(0, 0, 0, 0);
         if ((cells & 1) && threadIdx.x == 0) acc[cells - 1] = 0ull;
     }
-    slab_twiddles<T>(twz, twy, F);
-    // segment d: the records of plane x0 - n/2 + d (mod K0), whose atoms reach
-    // plane x0 through their x weight d
-    if (threadIdx.x == 0) {
-        int pref = 0;
-        for (int d = 0; d < n; ++d) {
-            const int cx = imod(x0 - n / 2 + d, K0);
-            const int beg = F.colstart[cx * K1];
-            seg_begin[d] = beg;
-            seg_pref[d] = pref;
-            pref += F.colstart[(cx + 1) * K1] - beg;
+    if (threadIdx.x < 64) {
+        int v = slen;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if ((int)threadIdx.x >= o) v += t;
         }
-        seg_pref[n] = pref;
+        if ((int)threadIdx.x < n) {
+            seg_begin[threadIdx.x] = sbeg;
+            seg_pref[threadIdx.x] = v - slen;
+        }
+        if ((int)threadIdx.x == n - 1) seg_pref[n] = v;
     }
     __syncthreads();
     const int units = seg_pref[n];
@@ -178,6 +192,9 @@ spread_slab_kernel(SPME_FRAME_ARGS, SplineArgs<T> sp) {
         for (int q = 0; q < kPre; ++q)
             if (dq[q] >= 0) atom(v[q], dq[q]);
     }
+    // the twiddles held since the start (K1, K2 <= 1024 = NT)
+    if ((int)threadIdx.x < K2) twz[threadIdx.x] = rtz;
+    if ((int)threadIdx.x < K1) twy[threadIdx.x] = rty;
     __syncthreads();
     const double inv = 1.0 / scale;
     auto cellv = [&](int y, int z) -> T {
