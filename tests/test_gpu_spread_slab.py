@@ -157,3 +157,51 @@ def test_fused_spreading_availability(precision, fixed, K, order, expect):
     info = s.info()
     say(precision, fixed, K, order, "slab", info["slab"])
     assert info["slab"] == expect
+
+
+def test_batch_mixing_128_planes_with_other_grids():
+    """A 128 x 128 x 128 frame (alone: the register-line slab transforms and
+    their fused spreading) batched with a 48^3 frame: the launch runs the
+    LDS-staged slab kernels for both, so each frame matches its evaluation
+    alone to FP32 rounding (the transforms round in another order), and the
+    oracle."""
+    dev = torch.device("cuda:0")
+    systems = [make_sys([40., 40., 40., 0., 0., 0.], (128, 128, 128), 4, 6000, 41),
+               make_sys([30., 31., 29., 0., 0., 0.], (48, 48, 48), 4, 2000, 42)]
+    handles, bufs = [], []
+    for sy in systems:
+        h = ew().Sfac(sy["L6"], np.array(sy["K"]), 4, precision="fp32")
+        h.set_A(ew().ewald_kernel(sy["beta"]), 1.0)
+        assert h.info()["slab"] == 2
+        handles.append(h)
+        n = len(sy["w"])
+        bufs.append(dict(w=torch.tensor(sy["w"], device=dev), x=torch.tensor(sy["x"], device=dev),
+                         out=torch.zeros(8, dtype=torch.float64, device=dev),
+                         dx=torch.zeros((n, 3), dtype=torch.float64, device=dev),
+                         phi=torch.zeros(n, dtype=torch.float64, device=dev)))
+    alone = [h.evaluate(sy["w"], sy["x"]) for h, sy in zip(handles, systems)]
+    batch = ew().SfacBatch(handles)
+    for f, (sy, b) in enumerate(zip(systems, bufs)):
+        batch.set(f, len(sy["w"]), b["w"].data_ptr(), b["x"].data_ptr(), b["out"].data_ptr(),
+                  b["dx"].data_ptr(), b["phi"].data_ptr())
+    torch.cuda.synchronize()
+    for rep in range(2):
+        batch.eval(None)
+        batch.sync()
+        torch.cuda.synchronize()
+        for f, (a, b) in enumerate(zip(alone, bufs)):
+            o = b["out"].cpu().numpy()
+            dx = b["dx"].cpu().numpy()
+            say("mixed batch", f, rep, abs(o[0] - a[0]) / abs(a[0]), rel_rms(dx, a[2]))
+            assert abs(o[0] - a[0]) <= 1e-6 * abs(a[0]), (rep, f)
+            assert rel(o[1:7], a[1]) <= 1e-5
+            assert rel_rms(dx, a[2]) <= 1e-5
+            assert rel_rms(b["phi"].cpu().numpy(), a[3]) <= 1e-5
+    for f, sy in enumerate(systems):
+        o = OracleSfac(sy["L6"], sy["K"], 4)
+        o.set_A(ewald_rad(sy["beta"]), 1.0)
+        E0, v0, dx0, phi0 = o.evaluate(sy["w"], sy["x"])
+        E = bufs[f]["out"][0].item()
+        assert abs(E - E0) <= 1e-6 * abs(E0)
+        assert rel_rms(bufs[f]["dx"].cpu().numpy(), dx0) <= 1e-5
+    batch.close()
