@@ -1883,6 +1883,58 @@ void Batch::rebind(const std::vector<Engine *> &frames) {
     graph_exec_ = nullptr;
     graph_ = nullptr;
     dirty_ = true;
+    if (lanes_ > 1) build_parts_();
+}
+
+void Batch::set_lanes(int k) {
+    if (k < 1 || k > 4) throw std::invalid_argument("lanes must be 1 to 4");
+    lanes_ = k;
+    build_parts_();
+}
+
+// groups of consecutive frames, as even in frame count as the division
+// allows; the group objects (and their device arrays) are kept across
+// rebinds of the same or fewer frames
+void Batch::build_parts_() {
+    const int nf = (int)frames_.size();
+    const int k = std::min(lanes_, nf);
+    if (k <= 1) {
+        parts_.clear();
+        part_begin_.clear();
+        return;
+    }
+    part_begin_.resize(k + 1);
+    for (int g = 0; g <= k; ++g) part_begin_[g] = (int)((long)g * nf / k);
+    parts_.resize(k);
+    for (int g = 0; g < k; ++g) {
+        const std::vector<Engine *> fr(frames_.begin() + part_begin_[g],
+                                       frames_.begin() + part_begin_[g + 1]);
+        if (parts_[g]) parts_[g]->rebind(fr);
+        else parts_[g].reset(new Batch(fr));
+    }
+    for (int f = 0; f < nf; ++f) part_atoms_(f);
+    if (!fork_) HIPC(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    while ((int)join_.size() < k) {
+        hipEvent_t e = nullptr;
+        HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        join_.push_back(e);
+    }
+}
+
+void Batch::part_atoms_(int f) {
+    if (parts_.empty()) return;
+    int g = 0;
+    while (f >= part_begin_[g + 1]) ++g;
+    Batch &p = *parts_[g];
+    const int i = f - part_begin_[g];
+    p.atoms_[i] = atoms_[f];
+    p.apply_atoms_(i);
+    p.dirty_ = true;
+}
+
+void Batch::synchronize() {
+    HIPC(hipStreamSynchronize(default_stream()));
+    for (const auto &p : parts_) HIPC(hipStreamSynchronize(p->default_stream()));
 }
 
 void Batch::apply_atoms_(int f) {
@@ -1958,6 +2010,9 @@ void Batch::refresh_variant_() {
 }
 
 Batch::~Batch() {
+    parts_.clear();
+    if (fork_) (void)hipEventDestroy(fork_);
+    for (hipEvent_t e : join_) (void)hipEventDestroy(e);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     (void)hipFree(d_descs_);
@@ -1982,6 +2037,7 @@ void Batch::set_atoms(int f, int n, const double *w_d, const double *x_d, double
     atoms_[f] = Atoms{n, w_d, x_d, out_d, dx_d, phi_d};
     apply_atoms_(f);
     dirty_ = true;
+    part_atoms_(f);
 }
 
 void Batch::upload_(hipStream_t st) {
@@ -2014,6 +2070,25 @@ void Batch::upload_(hipStream_t st) {
 
 void Batch::evaluate(hipStream_t st) {
     if (!st) st = frames_[0]->stream();  // the legacy null stream cannot be captured
+    if (!parts_.empty() && !frames_[0]->profiling()) {
+        // group 0 on st, group g on its own stream between a fork and a join
+        HIPC(hipEventRecord(fork_, st));
+        for (size_t g = 1; g < parts_.size(); ++g) {
+            const hipStream_t s = parts_[g]->default_stream();
+            if (s != st) HIPC(hipStreamWaitEvent(s, fork_, 0));
+        }
+        for (size_t g = 0; g < parts_.size(); ++g) {
+            parts_[g]->use_graph = use_graph;
+            parts_[g]->evaluate(g == 0 ? st : parts_[g]->default_stream());
+        }
+        for (size_t g = 1; g < parts_.size(); ++g) {
+            const hipStream_t s = parts_[g]->default_stream();
+            if (s == st) continue;
+            HIPC(hipEventRecord(join_[g], s));
+            HIPC(hipStreamWaitEvent(st, join_[g], 0));
+        }
+        return;
+    }
     // a frame whose engine changed its box, table, launch shapes or workspace
     // since the snapshot (set_L, set_A, autotuning, growth) is taken anew
     bool changed = false;

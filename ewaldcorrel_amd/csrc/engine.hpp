@@ -8,6 +8,7 @@
 
 #include <array>
 #include <initializer_list>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -357,6 +358,18 @@ public:
                    double *dx_d, double *phi_d);
     // all frames, one launch per stage; st == nullptr: frame 0's stream
     void evaluate(hipStream_t st);
+    // the frames as k groups of consecutive frames (k = 1..4, at most one
+    // group per frame), each replayed on its own stream: the first group on
+    // the evaluation's stream, group g on the stream of its first frame's
+    // engine, forked from the evaluation's stream and joined back into it by
+    // events, so that the kernels of one group fill the CUs that the other
+    // leaves idle.  Every frame runs the same kernels as in one group; k = 1:
+    // one launch per stage for every frame.  A profiling frame 0 evaluates
+    // the frames as one group (its timings cover every frame).
+    void set_lanes(int k);
+    int lanes() const { return lanes_; }
+    // waits for the default stream and the stream of every group
+    void synchronize();
     int size() const { return (int)frames_.size(); }
     const std::vector<Engine *> &frames() const { return frames_; }
     hipStream_t default_stream() const { return frames_[0]->stream(); }
@@ -393,6 +406,14 @@ private:
     int xinf_ = 0;                          // influence form of the fused x launches
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
+    // groups of frames on their own streams (lanes_ > 1)
+    void build_parts_();
+    void part_atoms_(int f);  // the group holding frame f takes atoms_[f]
+    int lanes_ = 1;
+    std::vector<std::unique_ptr<Batch>> parts_;
+    std::vector<int> part_begin_;  // first frame of each group, then nf
+    hipEvent_t fork_ = nullptr;
+    std::vector<hipEvent_t> join_;
 };
 
 // Process-wide record of tuned launch shapes, keyed by grid, order and

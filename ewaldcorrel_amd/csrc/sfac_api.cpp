@@ -435,10 +435,24 @@ int sfac_batch_eval(void *b, void *stream) {
 int sfac_batch_sync(void *b) {
     return guard([&] {
         if (!b) throw std::invalid_argument("NULL batch");
-        spme::hip_check(
-            hipStreamSynchronize(reinterpret_cast<spme::Batch *>(b)->default_stream()),
-            "hipStreamSynchronize");
+        reinterpret_cast<spme::Batch *>(b)->synchronize();
     });
+}
+
+int sfac_batch_set_lanes(void *b, int lanes) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        reinterpret_cast<spme::Batch *>(b)->set_lanes(lanes);
+    });
+}
+
+int sfac_batch_lanes(void *b) {
+    int k = 0;
+    const int rc = guard([&] {
+        if (!b) throw std::invalid_argument("NULL batch");
+        k = reinterpret_cast<spme::Batch *>(b)->lanes();
+    });
+    return rc == 0 ? k : rc;
 }
 
 int sfac_batch_set_graph(void *b, int enable) {
@@ -575,7 +589,7 @@ int sfac_batch_autotune(void *bp, int reps, char *report, int report_len) {
         if (!bp) throw std::invalid_argument("NULL batch");
         spme::Batch *b = reinterpret_cast<spme::Batch *>(bp);
         rep = spme::autotune(b->frames(), *b, b->default_stream(), reps);
-        spme::hip_check(hipStreamSynchronize(b->default_stream()), "hipStreamSynchronize");
+        b->synchronize();
     });
     if (rc != 0) return rc;
     put_text(rep, report, report_len);

@@ -1,6 +1,6 @@
 // Autotuner of launch shapes: FFT radix order per axis, transform path,
-// spreading tile and gather tile, timed on the device over whole batched
-// evaluations (declared in engine.hpp).
+// spreading tile and gather tile, and the batch's groups on streams, timed
+// on the device over whole batched evaluations (declared in engine.hpp).
 #include <stdio.h>
 
 #include <algorithm>
@@ -161,6 +161,28 @@ std::string autotune(const std::vector<Engine *> &frames, Batch &b, hipStream_t 
         }
         for (Engine *e : cls) e->apply_tune(cur);
         tune_remember(*e0, cur);
+    }
+    // the batch as two groups of frames on two streams, or as one group (a
+    // property of this batch, not of the frames' classes), kept as the
+    // shapes are
+    if (frames.size() >= 2) {
+        const int was = b.lanes(), k = was == 1 ? 2 : 1;
+        b.set_lanes(k);
+        double t = time();
+        bool keep = t < 0.99 * best;
+        if (keep) {
+            b.set_lanes(was);
+            best = std::min(best, time());
+            b.set_lanes(k);
+            t = std::max(t, time());
+            keep = t < 0.99 * best;
+        }
+        snprintf(line, sizeof(line), "batch in %d group%s on %s: %.4f ms%s\n", k,
+                 k == 1 ? "" : "s", k == 1 ? "one stream" : "their own streams", t,
+                 keep ? " (kept)" : "");
+        rep += line;
+        if (keep) best = t;
+        else b.set_lanes(was);
     }
     b.rebind(frames);
     snprintf(line, sizeof(line), "end %.4f ms per batch evaluation (start %.4f)\n", best,

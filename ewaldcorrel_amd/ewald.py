@@ -127,6 +127,8 @@ def describe_sfac():
         int_fn(lib.sfac_batch_eval, c_void_p, c_void_p)
         int_fn(lib.sfac_batch_sync, c_void_p)
         int_fn(lib.sfac_batch_set_graph, c_void_p, c_int)
+        int_fn(lib.sfac_batch_set_lanes, c_void_p, c_int)
+        int_fn(lib.sfac_batch_lanes, c_void_p)
         int_fn(lib.sfac_batch_destroy, c_void_p)
         decl_fn(lib.sfac_server_create, c_int, c_double, c_int, c_int, c_void_p)
         decl_fn(lib.sfac_server_submit, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
@@ -447,6 +449,17 @@ class SfacBatch:
 
     def set_graph(self, enable=True):
         _check(self._lib.sfac_batch_set_graph(self._b, int(bool(enable))), "batch set_graph")
+
+    def set_lanes(self, k):
+        """Evaluate the frames as `k` groups of consecutive frames (1-4),
+        each on its own stream (the first on the evaluation's stream, group g
+        on its first frame's stream, forked and joined by events)."""
+        _check(self._lib.sfac_batch_set_lanes(self._b, int(k)), "sfac_batch_set_lanes")
+
+    def lanes(self):
+        k = self._lib.sfac_batch_lanes(self._b)
+        _check(min(k, 0), "sfac_batch_lanes")
+        return k
 
     def autotune(self, reps=10):
         """Time and choose launch shapes for the frames (after set() of every

@@ -117,6 +117,20 @@ int sfac_batch_sync(void *b);
 int sfac_batch_set_graph(void *b, int enable);
 int sfac_batch_destroy(void *b);
 
+/* sfac_batch_set_lanes: the batch's frames as `lanes` groups of consecutive
+ * frames (1 to 4, at most one group per frame), each replayed on its own
+ * stream: the first group on the evaluation's stream, group g on the stream
+ * of its first frame's handle, forked from and joined back into the
+ * evaluation's stream by events, so that one group's kernels fill the CUs
+ * the other leaves idle.  Each frame's results are those of one group; 1
+ * (the default) launches each stage once for every frame.
+ * sfac_batch_autotune tries the other of 1 and 2 and keeps it when faster.
+ * sfac_batch_lanes returns the current count (or an error code).
+ * sfac_batch_sync waits for every group's stream.  No counterpart in the
+ * reference. */
+int sfac_batch_set_lanes(void *b, int lanes);
+int sfac_batch_lanes(void *b);
+
 /* Structure-factor accumulation on the device, asynchronous on the
  * handle's stream (the callers of get_S: run.py:36-47 sums |S|^2 over
  * frames, gmx_ewald.py:78-107 sums S_i conj(S_j) over species pairs,
