@@ -27,6 +27,30 @@ template <> struct cplx_t<float> { using type = float2; };
 template <> struct cplx_t<double> { using type = double2; };
 template <typename T> using cplx = typename cplx_t<T>::type;
 
+// Global-memory views of generic pointers.  An access through a generic
+// pointer compiles to a flat instruction, which counts against lgkmcnt as
+// well as vmcnt, so each LDS wait after it waits for that memory access too
+// (a loop of LDS reads feeding stores then waits for every store in turn);
+// through these views the access is a global instruction (vmcnt only).
+// 8-byte words move as native two-element vectors (the HIP vector types
+// have no assignment in another address space).
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+template <typename T> __device__ __forceinline__ gmem<T> *gptr(T *p) { return (gmem<T> *)p; }
+template <typename T> __device__ __forceinline__ const gmem<T> *gptr(const T *p) {
+    return (const gmem<T> *)p;
+}
+__device__ __forceinline__ uint2 gload(const uint2 *p, long i) {
+    const u32x2_t v = ((const gmem<u32x2_t> *)p)[i];
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(uint2 *p, long i, uint2 v) {
+    u32x2_t w;
+    w.x = v.x;
+    w.y = v.y;
+    ((gmem<u32x2_t> *)p)[i] = w;
+}
+
 // Box: lower-triangular inverse iL (rows = reciprocal basis) and 1/V.
 // L6 = [a_x, b_y, c_z, b_x, c_x, c_y] (sfac.hpp:37-52).
 struct CellArgs {

@@ -1186,10 +1186,22 @@ fft_x_reg_kernel(SPME_FRAME_ARGS) {
     unsigned e1 = e0;
     asm volatile("" : "+v"(e1));
     if (valid) {
+        if constexpr (std::is_same<T, float>::value) {
+            // one 8-byte store per element (with the offset opaque, the
+            // compiler cannot merge the two 4-byte stores of an element)
+            typedef __attribute__((address_space(1))) unsigned long long GU;
+            GU *grid2 = (GU *)F.grid;
+            const unsigned h1 = e1 >> 1, hs = es >> 1;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            grid[e1 + (unsigned)r * es] = v[r].x;
-            grid[e1 + (unsigned)r * es + 1u] = v[r].y;
+            for (int r = 0; r < 16; ++r)
+                grid2[h1 + (unsigned)r * hs] =
+                    ((unsigned long long)__float_as_uint(v[r].y) << 32) | __float_as_uint(v[r].x);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                grid[e1 + (unsigned)r * es] = v[r].x;
+                grid[e1 + (unsigned)r * es + 1u] = v[r].y;
+            }
         }
     }
     // the workgroup's partial: its wavefront sums in order (made visible by

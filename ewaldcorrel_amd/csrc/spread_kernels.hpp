@@ -66,7 +66,8 @@ __device__ __forceinline__ T spline_one(const SplineArgs<T> &sp, int n, T x, int
 __device__ __forceinline__ void atom_cell(const FrameDesc &F, int a, int c[3], double f[3]) {
 #pragma clang fp contract(off)
     double s0, s1, s2;
-    scale_wrap(F.cell, F.x[3 * a], F.x[3 * a + 1], F.x[3 * a + 2], s0, s1, s2);
+    const gmem<double> *gx = gptr(F.x);
+    scale_wrap(F.cell, gx[3 * a], gx[3 * a + 1], gx[3 * a + 2], s0, s1, s2);
     const double u0 = s0 * F.K0, u1 = s1 * F.K1, u2 = s2 * F.K2;
     c[0] = (int)u0;
     c[1] = (int)u1;
@@ -112,7 +113,7 @@ __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx,
                       "AtomRec<float> layout");
         v[0] = make_uint2(__float_as_uint((float)f[0]), __float_as_uint((float)f[1]));
         v[1] = make_uint2(__float_as_uint((float)f[2]), cell);
-        v[2] = make_uint2(__float_as_uint((float)F.w[a]), (unsigned)a);
+        v[2] = make_uint2(__float_as_uint((float)gptr(F.w)[a]), (unsigned)a);
     } else {
         static_assert(offsetof(AtomRec<double>, w) == 24 && offsetof(AtomRec<double>, cell) == 32,
                       "AtomRec<double> layout");
@@ -121,7 +122,7 @@ __device__ __forceinline__ void make_rec_raw(const FrameDesc &F, int a, int &cx,
             const unsigned long long fb = (unsigned long long)__double_as_longlong(f[d]);
             v[d] = make_uint2((unsigned)fb, (unsigned)(fb >> 32));
         }
-        const unsigned long long wb = (unsigned long long)__double_as_longlong(F.w[a]);
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(gptr(F.w)[a]);
         v[3] = make_uint2((unsigned)wb, (unsigned)(wb >> 32));
         v[4] = make_uint2(cell, (unsigned)a);
         v[5] = make_uint2(0u, 0u);
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         if (a < F.n) {
             make_rec_raw<T>(F, a, cx[q], r[q]);
             // the charge as the record holds it (FP32 records: rounded to FP32)
-            wm = fmax(wm, fabs((double)(T)F.w[a]));
+            wm = fmax(wm, fabs((double)(T)gptr(F.w)[a]));
             lr[q] = atomicAdd(&pl[cx[q]], 1);
         }
     }
@@ -209,9 +210,9 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         const int nbin = (F.n + BA - 1) / BA;
         if (p < K0) {
             pl[p] = st;
-            F.binstart[(long)p * nbin + b] = st;
+            gptr(F.binstart)[(long)p * nbin + b] = st;
         }
-        if (p == 0) F.binstart[(long)K0 * nbin + b] = tot;
+        if (p == 0) gptr(F.binstart)[(long)K0 * nbin + b] = tot;
     }
     __syncthreads();
     uint2 *stage = reinterpret_cast<uint2 *>(smem_raw);
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
     __syncthreads();
     const int nb = min(BA, F.n - b * BA);
     uint2 *dst = reinterpret_cast<uint2 *>(F.bins) + (long)b * BA * RW;
-    for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = stage[i];
+    for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) gstore(dst, i, stage[i]);
     if (F.fixed) {  // one partial maximum per workgroup
         for (int off = 32; off > 0; off >>= 1) wm = fmax(wm, __shfl_xor(wm, off));
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(kBinThreads) bin_kernel(SPME_FRAME_ARGS) {
         if (threadIdx.x == 0) {
             double m = 0.0;
             for (int k = 0; k < kBinThreads / 64; ++k) m = fmax(m, red[k]);
-            F.wmax[b] = m;
+            gptr(F.wmax)[b] = m;
         }
     }
 }
@@ -295,7 +296,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nbin = (F.n + BA - 1) / BA;
     // the plane's start and length in every chunk: rows cx and cx + 1
-    const int *row0 = F.binstart + (long)cx * nbin, *row1 = row0 + nbin;
+    const gmem<int> *row0 = gptr(F.binstart) + (long)cx * nbin, *row1 = row0 + nbin;
     const uint2 *bins = reinterpret_cast<const uint2 *>(F.bins);
     // atoms of the planes before this one (part), this plane's run in chunk
     // k = thread (c0, when there are at most NT chunks) and the runs' wave sums
@@ -365,7 +366,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             const int i = q * NT + (int)threadIdx.x;
             const long g = i < nb ? rmap[i] : 0;
 #pragma unroll
-            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? bins[g * RW + w] : make_uint2(0, 0);
+            for (int w = 0; w < RW; ++w) r[q][w] = i < nb ? gload(bins, g * RW + w) : make_uint2(0, 0);
         }
 #pragma unroll
         for (int q = 0; q < kSortRegs; ++q)
@@ -411,13 +412,13 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
         const int e0 = coff + ci - sm;  // start of column y0 within the plane
         if (y0 < K1) {
             cur[y0] = e0;
-            F.colstart[cx * K1 + y0] = pre + e0;
+            gptr(F.colstart)[cx * K1 + y0] = pre + e0;
         }
         if (y0 + 1 < K1) {
             cur[y0 + 1] = e0 + v0;
-            F.colstart[cx * K1 + y0 + 1] = pre + e0 + v0;
+            gptr(F.colstart)[cx * K1 + y0 + 1] = pre + e0 + v0;
         }
-        if (cx == K0 - 1 && threadIdx.x == 0) F.colstart[K0 * K1] = pre + nb;
+        if (cx == K0 - 1 && threadIdx.x == 0) gptr(F.colstart)[K0 * K1] = pre + nb;
     }
     __syncthreads();
     uint2 *dst = reinterpret_cast<uint2 *>(F.recs) + (long)pre * RW;
@@ -431,7 +432,7 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
                 for (int w = 0; w < RW; ++w) sbuf[slot * RW + w] = r[q][w];
             }
         __syncthreads();  // contiguous write-out, 16 bytes per lane
-        for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) dst[i] = sbuf[i];
+        for (int i = threadIdx.x; i < nb * RW; i += blockDim.x) gstore(dst, i, sbuf[i]);
     } else if (table) {
         // every record straight to its place, window by window
         for (int i0 = 0; i0 < nb; i0 += WIN) {
@@ -444,11 +445,11 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
                     const long g = rmap[i - i0];
                     uint2 v[RW];
 #pragma unroll
-                    for (int w = 0; w < RW; ++w) v[w] = bins[g * RW + w];
+                    for (int w = 0; w < RW; ++w) v[w] = gload(bins, g * RW + w);
                     // the cell by a load of its own (as below)
                     const long slot = atomicAdd(&cur[(rec_cell<T>(bins, (int)g) >> 10) & 1023], 1);
 #pragma unroll
-                    for (int w = 0; w < RW; ++w) dst[slot * RW + w] = v[w];
+                    for (int w = 0; w < RW; ++w) gstore(dst, slot * RW + w, v[w]);
                 }
             }
             __syncthreads();  // the map is rewritten for the next window
@@ -460,24 +461,24 @@ __global__ void __launch_bounds__(kSortThreads) sort_kernel(SPME_FRAME_ARGS) {
             for (int j = lane; j < c; j += 64) {
                 uint2 v[RW];
 #pragma unroll
-                for (int w = 0; w < RW; ++w) v[w] = run[(long)j * RW + w];
+                for (int w = 0; w < RW; ++w) v[w] = gload(run, (long)j * RW + w);
                 // the cell by a load of its own: taken from the loaded words,
                 // the compiler moved the rest of them through scratch memory
                 const long slot = atomicAdd(&cur[(rec_cell<T>(run, j) >> 10) & 1023], 1);
 #pragma unroll
-                for (int w = 0; w < RW; ++w) dst[slot * RW + w] = v[w];
+                for (int w = 0; w < RW; ++w) gstore(dst, slot * RW + w, v[w]);
             }
         }
     }
     if (cx == 0 && F.fixed) {  // the fixed-point scale, from the bin partial maxima
         double m = 0.0;
-        for (int i = threadIdx.x; i < nbin; i += blockDim.x) m = fmax(m, F.wmax[i]);
+        for (int i = threadIdx.x; i < nbin; i += blockDim.x) m = fmax(m, gptr(F.wmax)[i]);
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
         if (lane == 0) wred[wv] = m;
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int k = 0; k < NW; ++k) m = fmax(m, wred[k]);
-            *F.fxscale = fixed_scale(F.n, m, F.mmax3, F.fx32);
+            *gptr(F.fxscale) = fixed_scale(F.n, m, F.mmax3, F.fx32);
         }
     }
 }
