@@ -32,10 +32,11 @@ template <typename T> using cplx = typename cplx_t<T>::type;
 // well as vmcnt, so each LDS wait after it waits for that memory access too
 // (a loop of LDS reads feeding stores then waits for every store in turn);
 // through these views the access is a global instruction (vmcnt only).
-// 8-byte words move as native two-element vectors (the HIP vector types
-// have no assignment in another address space).
+// 8- and 16-byte words move as native vectors (the HIP vector types have
+// no assignment in another address space).
 template <typename T> using gmem = __attribute__((address_space(1))) T;
 typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 template <typename T> __device__ __forceinline__ gmem<T> *gptr(T *p) { return (gmem<T> *)p; }
 template <typename T> __device__ __forceinline__ const gmem<T> *gptr(const T *p) {
     return (const gmem<T> *)p;

@@ -215,9 +215,9 @@ __device__ __forceinline__ void slab128_fwd_lines(const FrameDesc &F, int x, flo
             for (int jj = 0; jj < 8; ++jj) S[((2 * jy + h) + 16 * jj) * RS + ly] = u[h][jj];
     }
     lds_sync();
-    // the half-complex plane out, 16 B per lane
-    uint4 *out = reinterpret_cast<uint4 *>((float2 *)F.grid + (long)x * N * RS);
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(S);
+    // the half-complex plane out, 16 B per lane (global stores)
+    gmem<u32x4_t> *out = (gmem<u32x4_t> *)((float2 *)F.grid + (long)x * N * RS);
+    const u32x4_t *s4 = reinterpret_cast<const u32x4_t *>(S);
     for (int l = t; l < N * RS / 2; l += NT) out[l] = s4[l];
 }
 
@@ -340,7 +340,7 @@ fft_yz_inv128_kernel(SPME_FRAME_ARGS, int finish) {
         for (int r = 0; r < 16; ++r) lb[jz + 8 * r] = v[r];
     }
     lds_sync();
-    float *g = (float *)F.grid + (long)x * N * 2 * RS;
+    gmem<float> *g = gptr((float *)F.grid) + (long)x * N * 2 * RS;  // global stores
     for (int l = t; l < N / 2 * N; l += NT) {
         const int c = l >> 7, z = l & (N - 1);
         const float2 w = reg[c * LS + z];
