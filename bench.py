@@ -275,6 +275,7 @@ def run_precision(systems, precision, steps, warmup, profile, tune=True, split=(
         for sb in subs:
             sb.close()
     bat["table_mode"] = handles[0].info()["table_mode"]
+    bat["lanes"] = batch.lanes()
     bat["E"] = [float(inp.out[0].item()) for inp in inputs]
     bat["vir"] = [inp.out[1:7].cpu().numpy().copy() for inp in inputs]
     bat["dx0"] = inputs[0].dx.cpu().numpy()
@@ -765,7 +766,8 @@ def main():
                                 "influence, energy/virial sums and reductions; int64 "
                                 "fixed-point grid accumulation",
                    "influence": ("closed form from the indices (no table read)"
-                                 if b32["table_mode"] == 1 else "table read from HBM")},
+                                 if b32["table_mode"] == 1 else "table read from HBM"),
+                   "batch_groups": b32["lanes"]},
         "set_A_ms": {"median": float(np.median(SET_A_MS)), "max": float(np.max(SET_A_MS)),
                      "what": "influence table built on the device, checked against "
                              "the closed form; per frame"},
