@@ -205,3 +205,39 @@ def test_autotune_tries_groups_and_keeps_results():
         assert rel(out[1:7], v0) <= 1e-5, f
         assert rel_rms(bufs[f]["dx"].cpu().numpy(), dx0) <= 1e-5, f
     batch.close()
+
+
+def test_lanes_follow_a_box_change():
+    """set_L (and the table it rebuilds) on frames of the first and the last
+    group between two grouped evaluations: each group takes its frames'
+    new descriptors, so every frame matches its evaluation alone in the new
+    box bitwise."""
+    dev = torch.device("cuda:0")
+    systems, handles, bufs = make_batch("fp32", dev)
+    batch = ew().SfacBatch(handles)
+    for f, (sy, b) in enumerate(zip(systems, bufs)):
+        bind(batch, f, sy, b)
+    batch.set_lanes(2)
+    batch.eval(None)
+    batch.sync()
+    for f, scale in ((0, 1.03), (4, 0.97)):
+        L6 = systems[f]["L6"] * scale
+        handles[f].set_L(L6)
+        systems[f]["L6"] = L6
+    alone = [h.evaluate(sy["w"], sy["x"]) for h, sy in zip(handles, systems)]
+    for rep in range(2):
+        clear(bufs)
+        batch.eval(None)
+        batch.sync()
+        check_alone(bufs, alone, ("box change", rep))
+    # and the oracle in the new box for the changed frames
+    for f in (0, 4):
+        sy = systems[f]
+        o = OracleSfac(sy["L6"], sy["K"], 4)
+        o.set_A(ewald_rad(sy["beta"]), 1.0)
+        E0, v0, dx0, phi0 = o.evaluate(sy["w"], sy["x"])
+        E = bufs[f]["out"][0].item()
+        say("box change frame", f, abs(E - E0) / abs(E0))
+        assert abs(E - E0) <= 1e-6 * abs(E0)
+        assert rel_rms(bufs[f]["dx"].cpu().numpy(), dx0) <= 1e-5
+    batch.close()
